@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Benchmark of the mode-q tensor-times-vector product on one MI355X.
+
+Workload (one "step"): the FP32 order-4 tensor 128^4 (2^28 elements, 1 GiB, U[-1,1] from a
+seed), contracted over every mode q = 1..4 in first-order and in last-order layout -- 8 products
+per step, inputs resident in HBM before timing starts.  `value` is the step's algorithmic bytes
+(per product N*4 + (N/n_q)*4 + n_q*4 = 1,082,130,944 B) divided by the step time, in GB/s.
+
+Also reported on the same line:
+  roofline      -- the product kernel with the largest share of the step time: algorithmic
+                   bytes per launch / its mean launch time (HIP events on the launch stream)
+                   against the 8 TB/s HBM peak; traffic from the committed rocprofv3 counters
+  per_case      -- GB/s and fraction of peak for each (q, layout)
+  read_ceiling  -- a pure 16-byte read stream over the same 1 GiB (the practical ceiling)
+  precision     -- BF16 and FP8-e4m3 storage of A: speedup over the FP32 kernel per case and the
+                   relative L2 error of C against the FP64 oracle on the original data (and on the
+                   dequantised data)
+  cpu_baseline  -- the reference's CPU path (restated in oracle/: OpenMP over slices + OpenBLAS
+                   gemv, par_loop/subtensor/all) on the host's physical cores, on the same inputs
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-precision]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+PEAK_BPS = 8.0e12
+SHAPE = [128, 128, 128, 128]
+N = 1 << 28
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def case_bytes(nq, elem=4, c_elem=4, extra=0):
+    return N * elem + (N // nq) * c_elem + nq * 4 + extra
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernel measured with rocprofv3 counters (committed
+    summary), or None."""
+    path = os.path.join(REPO, "profiles", "traffic_ttv_f32_128x4.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are out of scope (the mode-q product "
+              "shards over free modes with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ttv_amd
+    from ttv_amd.meta import generate_strides
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20250101)
+    log("generating A (2^28 FP32, U[-1,1]) on the GPU")
+    Aflat = torch.rand(N, generator=gen, device=dev, dtype=torch.float32).mul_(2).sub_(1)
+    layouts = {"first": [1, 2, 3, 4], "last": [4, 3, 2, 1]}
+    cases = []
+    for lname, pia in layouts.items():
+        wa = generate_strides(SHAPE, pia)
+        A = Aflat.as_strided(SHAPE, wa)
+        for q in range(1, 5):
+            b = torch.rand(128, generator=gen, device=dev, dtype=torch.float32).mul_(2).sub_(1)
+            info = ttv_amd.plan_info(q, SHAPE, wa, pia)
+            C = ttv_amd.ttv(q, A, b)             # allocates C once; reused in the timed loop
+            cases.append(dict(layout=lname, pia=pia, wa=wa, q=q, A=A, b=b, C=C, variant=info["variant"],
+                              kernel=ttv_amd.variant_names()[info["variant"]], bytes=case_bytes(128)))
+    torch.cuda.synchronize()
+
+    def run_case(c):
+        ttv_amd.ttv(c["q"], c["A"], c["b"], out=c["C"])
+
+    for _ in range(args.warmup):
+        for c in cases:
+            run_case(c)
+    torch.cuda.synchronize()
+
+    # timed region: K steps, per-case events on the launch stream
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
+          for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for s in range(args.steps):
+        for i, c in enumerate(cases):
+            ev[s][i][0].record(stream)
+            run_case(c)
+            ev[s][i][1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    step_bytes = sum(c["bytes"] for c in cases)
+    value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
+    for i, c in enumerate(cases):
+        ts = [ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(args.steps)]
+        c["ms"] = float(np.mean(ts))
+        c["ms_min"] = float(np.min(ts))
+        c["gbs"] = c["bytes"] / (c["ms"] * 1e-3) / 1e9
+    per_case = [dict(layout=c["layout"], q=c["q"], kernel=c["kernel"], ms=round(c["ms"], 4),
+                     gbs=round(c["gbs"], 1), frac=round(c["gbs"] * 1e9 / PEAK_BPS, 4)) for c in cases]
+    for pc in per_case:
+        log("  %-5s q=%d %-11s %8.4f ms  %7.1f GB/s  %.3f of peak" % (pc["layout"], pc["q"], pc["kernel"], pc["ms"],
+                                                                       pc["gbs"], pc["frac"]))
+    dom = max(cases, key=lambda c: c["ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3)
+    traffic = traffic_from_profiles()
+    roofline = dict(bound="hbm", achieved=round(achieved / 1e9, 1), peak=8000.0, unit="GB/s",
+                    frac=round(achieved / PEAK_BPS, 4),
+                    traffic=(traffic or {}).get("hbm_bytes_per_launch"),
+                    kernel="%s (%s-order, q=%d)" % (dom["kernel"], dom["layout"], dom["q"]),
+                    algorithmic_bytes_per_launch=dom["bytes"], launch_ms=round(dom["ms"], 4))
+
+    # practical ceiling: pure read stream over the same buffer
+    partial = torch.empty(256 * 8, dtype=torch.float32, device=dev)
+    L = ttv_amd.lib()
+    for _ in range(3):
+        ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
+                                        stream.cuda_stream))
+    rts = []
+    for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
+                                        stream.cuda_stream))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        rts.append(e0.elapsed_time(e1))
+    read_gbs = N * 4 / (np.median(rts) * 1e-3) / 1e9
+    log("read stream ceiling: %.1f GB/s (%.3f of peak)" % (read_gbs, read_gbs * 1e9 / PEAK_BPS))
+
+    out = {
+        "metric": "HBM GB/s (% peak) order-4 FP32 TTV; BF16/FP8 speedup at rel err<=1e-2",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (U[-1,1], seeded)",
+        "config": {"workload": "FP32 128^4 mode-q TTV, q=1..4 x {first,last}-order layouts (8 products per step)",
+                   "shape": SHAPE, "elements": N, "bytes_per_product": case_bytes(128)},
+        "frac_of_peak": round(value * 1e9 / PEAK_BPS, 4),
+        "min_case_frac": round(min(pc["frac"] for pc in per_case), 4),
+        "read_ceiling": {"gbs": round(read_gbs, 1), "frac": round(read_gbs * 1e9 / PEAK_BPS, 4),
+                         "value_over_ceiling": round(value / read_gbs, 4)},
+        "roofline": roofline,
+        "per_case": per_case,
+    }
+
+    # FP64 oracle results are needed for the precision errors; A goes to the host once
+    need_host = not (args.no_precision and args.no_cpu)
+    a_host = Aflat.cpu().numpy() if need_host else None
+
+    if not args.no_precision:
+        out["precision"] = precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log)
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_leg(np, a_host, cases, args, log)
+
+    print(json.dumps(out), flush=True)
+
+
+def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
+    from oracle import oracle as O
+    stream = torch.cuda.current_stream()
+    res = {"bf16": [], "fp8": []}
+    Abf = ttv_amd.quantize_bf16(Aflat)            # same strides (flat)
+    abf_host = None
+    for c in cases:
+        q, wa, pia = c["q"], c["wa"], c["pia"]
+        b = c["b"]
+        bh = b.cpu().numpy()
+        ref = O.naive(q, a_host, SHAPE, wa, bh, pia=pia)
+        rn = np.linalg.norm(ref)
+        c32 = c["C"].as_strided([c["C"].numel()], [1]).cpu().numpy().astype(np.float64)
+        err32 = float(np.linalg.norm(c32 - ref) / rn)
+        for fmt in ("bf16", "fp8"):
+            if fmt == "bf16":
+                Aq = Abf.as_strided(SHAPE, wa)
+                run = lambda: ttv_amd.ttv_bf16(q, Aq, b, out=Cq)  # noqa: E731
+                extra = 0
+                Cq = ttv_amd.ttv_bf16(q, Aq, b)
+                elem = 2
+            else:
+                codes, scales, fps = ttv_amd.quantize_fp8(q, Aflat.as_strided(SHAPE, wa))
+                Cq = ttv_amd.ttv_fp8(q, codes, scales, fps, b)
+                run = lambda: ttv_amd.ttv_fp8(q, codes, scales, fps, b, out=Cq)  # noqa: E731
+                extra = scales.numel() * 4
+                elem = 1
+            for _ in range(2):
+                run()
+            ts = []
+            for _ in range(max(5, args.steps)):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                run()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            ms = float(np.mean(ts))
+            got = Cq.as_strided([Cq.numel()], [1]).cpu().numpy().astype(np.float64)
+            err = float(np.linalg.norm(got - ref) / rn)
+            # error of the accumulation alone: FP64 oracle on the dequantised values
+            if fmt == "bf16":
+                if abf_host is None:
+                    abf_host = O.bf16_to_f32(Abf.cpu().numpy().view(np.uint16))
+                ref_q = O.naive(q, abf_host, SHAPE, wa, bh, pia=pia)
+            else:
+                # scales are constant along each fiber: C = scale[j // fps] * sum_k decode(code) b_k
+                table = O.e4m3_decode(np.arange(256, dtype=np.uint8))
+                deq = table[codes.as_strided([N], [1]).cpu().numpy()]
+                sh = scales.cpu().numpy().astype(np.float64)
+                ref_q = O.naive(q, deq, SHAPE, wa, bh, pia=pia)
+                ref_q *= sh[np.arange(ref_q.size) // fps]
+                del deq
+            err_acc = float(np.linalg.norm(got - ref_q) / np.linalg.norm(ref_q))
+            nbytes = case_bytes(128, elem=elem, extra=extra)
+            res[fmt].append(dict(layout=c["layout"], q=q, ms=round(ms, 4), speedup=round(c["ms"] / ms, 3),
+                                 gbs=round(nbytes / (ms * 1e-3) / 1e9, 1),
+                                 ideal_speedup=round(c["bytes"] / nbytes, 3),
+                                 rel_err_vs_fp64=err, rel_err_vs_fp64_dequantised=err_acc, fp32_rel_err=err32))
+            log("  %s %-5s q=%d %.4f ms  speedup %.2f (ideal %.2f)  rel err %.2e (accumulation %.2e)" % (
+                fmt, c["layout"], q, ms, c["ms"] / ms, c["bytes"] / nbytes, err, err_acc))
+            if fmt == "fp8":
+                del codes, scales
+            del Cq
+    summ = {}
+    for fmt in ("bf16", "fp8"):
+        sp = [r["speedup"] for r in res[fmt]]
+        summ[fmt] = dict(speedup_geomean=round(float(np.exp(np.mean(np.log(sp)))), 3),
+                         speedup_min=round(min(sp), 3),
+                         rel_err_max=max(r["rel_err_vs_fp64"] for r in res[fmt]),
+                         rel_err_dequantised_max=max(r["rel_err_vs_fp64_dequantised"] for r in res[fmt]),
+                         per_case=res[fmt])
+    return summ
+
+
+def cpu_leg(np, a_host, cases, args, log):
+    from oracle import oracle as O
+    blas = O.load_blas()
+    cores = O.number_cores()
+    t_total, nbytes, reps = 0.0, 0, 0
+    bh = [c["b"].cpu().numpy() for c in cases]
+    C = [np.zeros(N // 128, np.float32) for _ in cases]
+    # warm-up of every case, then whole steps until the time budget is used (at least one step)
+    for i, c in enumerate(cases):
+        O.log_cpu(c["q"], a_host, SHAPE, c["wa"], c["pia"], bh[i], c_out=C[i], threads=cores)
+    while reps < 1 or t_total < args.cpu_seconds:
+        t0 = time.perf_counter()
+        for i, c in enumerate(cases):
+            O.log_cpu(c["q"], a_host, SHAPE, c["wa"], c["pia"], bh[i], c_out=C[i], threads=cores)
+        t_total += time.perf_counter() - t0
+        nbytes += sum(c["bytes"] for c in cases)
+        reps += 1
+    gbs = nbytes / t_total / 1e9
+    log("cpu baseline: %.2f GB/s on %d cores (%s), %d steps in %.1f s" % (gbs, cores, blas or "no BLAS", reps, t_total))
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": cores, "kind": "port",
+            "blas": os.path.basename(blas) if blas else "none (reference's own OpenMP loops)",
+            "sample": "%d full steps (8 products of FP32 128^4, same inputs) in %.1f s" % (reps, t_total)}
+
+
+if __name__ == "__main__":
+    main()

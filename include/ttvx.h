@@ -1,0 +1,184 @@
+/*
+ * ttvx.h -- C-ABI of the MI355X (gfx950) mode-q tensor-times-vector library libttvx.so.
+ *
+ *   C(i_1..i_{q-1}, i_{q+1}..i_p) = sum_{i_q} A(i_1..i_p) * b(i_q)
+ *
+ * Conventions are those of the reference's low-level interface (include/tlib/ttv.h:54-92):
+ * q and layout tuples are one-based, extents/strides are indexed by mode, pi[r] is the mode
+ * at layout position r (r = 0 fastest), nb[0] must equal na[q-1], b is contiguous.
+ * Strides are counted in elements.  C is overwritten (beta = 0), never accumulated into.
+ *
+ * Every entry point returns TTVX_OK (0) or a nonzero status; the text of the last error of
+ * the calling thread is returned by ttvx_last_error().  Argument errors carry exactly the
+ * std::runtime_error messages of the reference (include/tlib/ttv.h:64-89 and
+ * include/tlib/detail/tensor_times_vector.h:160,164), so C++ and Python wrappers can
+ * re-throw them verbatim.
+ *
+ * Pointers: the synchronous entry points accept host or device memory for a, b and c (host
+ * buffers are staged through device memory and the call blocks until C is written back); the
+ * *_async entry points take device pointers only and enqueue on `stream` (a hipStream_t; NULL
+ * selects the default stream).
+ *
+ * No torch types and no C++ types cross this boundary.
+ */
+#ifndef TTVX_H
+#define TTVX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------- */
+#define TTVX_OK               0
+#define TTVX_ERR_ARGUMENT     1  /* argument validation failed (reference message)               */
+#define TTVX_ERR_LAYOUT       2  /* pia/pic inconsistent (tensor_times_vector.h:147-168 message)  */
+#define TTVX_ERR_DEVICE       3  /* HIP runtime error or no usable gfx950 device                 */
+#define TTVX_ERR_UNSUPPORTED  4  /* combination not provided by this build                       */
+
+/* ---- element types -------------------------------------------------------------------- */
+#define TTVX_F32      0  /* float  A, b, C                                      */
+#define TTVX_F64      1  /* double A, b, C                                      */
+#define TTVX_BF16     2  /* bfloat16 A (uint16_t storage), float b and C         */
+#define TTVX_FP8E4M3  3  /* OCP FP8 e4m3 A (uint8_t storage) + float scales; float b and C */
+
+/* ---- policy codes (include/tlib/detail/tags.h:21-63); hints only, every combination accepted */
+#define TTVX_EXEC_SEQ           0
+#define TTVX_EXEC_SEQ_BLAS      1
+#define TTVX_EXEC_PAR           2
+#define TTVX_EXEC_PAR_TASKLOOP  3
+#define TTVX_EXEC_PAR_TASK      4
+#define TTVX_EXEC_PAR_LOOP      5
+#define TTVX_EXEC_PAR_BLAS      6
+#define TTVX_EXEC_PAR_BLAS_LOOP 7
+#define TTVX_SLICE_SLICE        0
+#define TTVX_SLICE_SUBTENSOR    1
+#define TTVX_FUSE_NONE          0
+#define TTVX_FUSE_OUTER         1
+#define TTVX_FUSE_ALL           2
+
+/* ---- accumulation flags ------------------------------------------------------------- */
+#define TTVX_FLAG_KAHAN      1u  /* compensated (Kahan) summation along n_q                       */
+#define TTVX_FLAG_FP64_FINAL 2u  /* FP64 cross-lane / cross-block reduction (automatic when n_q > 2^20) */
+
+/* Text of the calling thread's last error ("" when the last call succeeded). */
+const char* ttvx_last_error(void);
+
+/* Library version as major*10000 + minor*100 + patch. */
+int ttvx_version(void);
+
+/* Number of visible HIP devices; 0 without a GPU.  Never fails. */
+int ttvx_device_count(void);
+
+/* ---- plain drop-in ------------------------------------------------------------------- */
+/* Replaces tlib::ttv::ttv<float,std::size_t,ep,sp,fp>(ep,sp,fp,q,p,a,na,wa,pia,b,nb,c,nc,wc,pic)
+ * (reference include/tlib/ttv.h:54-92, dispatching to include/tlib/detail/tensor_times_vector.h).
+ * ep/sp/fp: TTVX_EXEC_* / TTVX_SLICE_* / TTVX_FUSE_* codes.  Synchronous. */
+int ttvx_ttv_f32(int ep, int sp, int fp, size_t q, size_t p,
+                 const float* a, const size_t* na, const size_t* wa, const size_t* pia,
+                 const float* b, const size_t* nb,
+                 float* c, const size_t* nc, const size_t* wc, const size_t* pic);
+
+/* Same for double (tlib::ttv::ttv<double,...>, include/tlib/ttv.h:54-92). */
+int ttvx_ttv_f64(int ep, int sp, int fp, size_t q, size_t p,
+                 const double* a, const size_t* na, const size_t* wa, const size_t* pia,
+                 const double* b, const size_t* nb,
+                 double* c, const size_t* nc, const size_t* wc, const size_t* pic);
+
+/* Stream-ordered forms of the two above: device pointers only; flags = TTVX_FLAG_*. */
+int ttvx_ttv_f32_async(size_t q, size_t p,
+                       const float* a, const size_t* na, const size_t* wa, const size_t* pia,
+                       const float* b, const size_t* nb,
+                       float* c, const size_t* nc, const size_t* wc, const size_t* pic,
+                       unsigned flags, void* stream);
+int ttvx_ttv_f64_async(size_t q, size_t p,
+                       const double* a, const size_t* na, const size_t* wa, const size_t* pia,
+                       const double* b, const size_t* nb,
+                       double* c, const size_t* nc, const size_t* wc, const size_t* pic,
+                       unsigned flags, void* stream);
+
+/* ---- reduced-precision storage of A (no counterpart in the reference) ------------------ */
+/* A keeps its extents, strides and layout; only the element type changes.  Accumulation is
+ * FP32 (optionally Kahan), b and C are float.
+ *
+ * FP8 scales: the mode-q fibers of A are numbered in the order of C's dense memory offsets
+ * j (j = o*I + i in the folded [O][K][I] view).  Scale g multiplies the fibers
+ * g*fibers_per_scale <= j < (g+1)*fibers_per_scale; it is absmax(group)/448 (1 for an all-zero
+ * group) and the stored code of x is RNE-e4m3(x * (448/absmax)), saturated to +-448. */
+
+/* Round-to-nearest-even float -> bfloat16 over `count` contiguous elements. */
+int ttvx_quantize_bf16_async(size_t count, const float* src, uint16_t* dst, void* stream);
+
+/* Number of FP8 scales for (q, p, na, wa, pia) and the given group size, 0 on error. */
+size_t ttvx_fp8_num_scales(size_t q, size_t p, const size_t* na, const size_t* wa, const size_t* pia,
+                           size_t fibers_per_scale);
+
+/* Quantises FP32 A into e4m3 codes `aq` (same strides) and per-group `scales`. */
+int ttvx_quantize_fp8_async(size_t q, size_t p,
+                            const float* a, const size_t* na, const size_t* wa, const size_t* pia,
+                            uint8_t* aq, float* scales, size_t fibers_per_scale, void* stream);
+
+/* C = A x_q b with bfloat16 A. */
+int ttvx_ttv_bf16_async(size_t q, size_t p,
+                        const uint16_t* a, const size_t* na, const size_t* wa, const size_t* pia,
+                        const float* b, const size_t* nb,
+                        float* c, const size_t* nc, const size_t* wc, const size_t* pic,
+                        unsigned flags, void* stream);
+
+/* C = A x_q b with e4m3 A and its scales. */
+int ttvx_ttv_fp8_async(size_t q, size_t p,
+                       const uint8_t* a, const size_t* na, const size_t* wa, const size_t* pia,
+                       const float* scales, size_t fibers_per_scale,
+                       const float* b, const size_t* nb,
+                       float* c, const size_t* nc, const size_t* wc, const size_t* pic,
+                       unsigned flags, void* stream);
+
+/* ---- planning, variants and selection (no counterpart in the reference: the reference's
+ *      policy tags, tags.h:21-63, are chosen by the caller) ---------------------------- */
+#define TTVX_SELECT_HEURISTIC 0  /* fixed rule                                     */
+#define TTVX_SELECT_LEARNED   1  /* MLP selector (falls back to the rule without weights) */
+#define TTVX_SELECT_FORCED    2  /* the variant given to ttvx_set_selection      */
+
+#define TTVX_NUM_FEATURES 24
+
+/* Number of kernel variants and the name of variant v ("" if out of range). */
+int ttvx_num_variants(void);
+const char* ttvx_variant_name(int v);
+
+/* Selection mode for subsequent calls of this process (all threads). */
+int ttvx_set_selection(int mode, int forced_variant);
+
+/* Folded view of (q, p, na, wa, pia) for element type dtype:
+ *   out[0]=O, out[1]=K, out[2]=I, out[3]=1 if the folded 3-D view exists else 0,
+ *   out[4]=reference case number 1..8 (cases.h:24-36), out[5]=variant the current selection picks. */
+int ttvx_plan_info(int dtype, size_t q, size_t p, const size_t* na, const size_t* wa, const size_t* pia,
+                   int64_t* out6);
+
+/* 1 if variant v can run (q, p, na, wa, pia) in dtype, else 0. */
+int ttvx_variant_valid(int dtype, size_t q, size_t p, const size_t* na, const size_t* wa, const size_t* pia,
+                       int v);
+
+/* Selector input features of a problem (TTVX_NUM_FEATURES floats). */
+int ttvx_features(int dtype, size_t q, size_t p, const size_t* na, const size_t* wa, const size_t* pia,
+                  float* features);
+
+/* Number of floats in the selector MLP's parameter vector, and loading it (count 0 unloads).
+ * Layout: mean[24], inv_std[24], W1[64][24], b1[64], W2[64][64], b2[64], W3[12][64], b3[12];
+ * the network predicts log2(t_variant / t_fastest) for every variant. */
+size_t ttvx_selector_num_params(void);
+int ttvx_selector_set_weights(const float* params, size_t count);
+
+/* ---- diagnostics (no counterpart in the reference; used by the benchmark) -------------- */
+/* Pure read stream over `bytes` (16-byte aligned) writing one float per block to `partial`
+ * (partial_count >= number of blocks, at most 8 per CU): the practical HBM read ceiling. */
+int ttvx_bench_read(const void* src, size_t bytes, float* partial, size_t partial_count, void* stream);
+/* Plain copy of `bytes` (16-byte aligned): a known-size kernel for counter calibration. */
+int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* TTVX_H */

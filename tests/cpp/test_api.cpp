@@ -1,0 +1,204 @@
+// The drop-in C++ interfaces (include/tlib/ttv.h) on the GPU, written like the reference's
+// test/src/gtest_tlib_ttv.cpp:27-425: for every policy-tag combination the reference tests,
+// ranks 2..4, shapes {2,4,8}^p, every layout and every q, A is filled with the known-answer
+// pattern (consecutive integers along each mode-q fiber, :27-71) and b = 1, so that
+// C[j] = (nq^2 (2(j+1)-1) + nq)/2 exactly (:132).  Host std::vector buffers go through the
+// library's synchronous entry points (staged to the GPU).  Also runs the three example
+// programs' computations (example/interface1.cpp, interface2.cpp, interface3.cpp) and the
+// argument checks.  Exit status 0 = all checks passed.
+#include <tlib/ttv.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <functional>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace tlib::ttv;
+using vec = std::vector<std::size_t>;
+
+static long g_fail = 0, g_checks = 0;
+#define CHECK(cond, ...)                                                          \
+	do {                                                                          \
+		++g_checks;                                                               \
+		if(!(cond)) {                                                             \
+			if(++g_fail < 20) { std::fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__); std::fprintf(stderr, __VA_ARGS__); std::fprintf(stderr, "\n"); } \
+		}                                                                         \
+	} while(0)
+
+// fills every mode-q fiber with consecutive integers, fibers in C's storage order
+template<class T>
+static void known_init(std::size_t q, std::vector<T>& a, vec const& na, vec const& wa, vec const& pia)
+{
+	std::size_t const p = na.size();
+	std::size_t const qpos = detail::inverse_mode(pia.begin(), pia.end(), q) - 1;
+	std::size_t nfib = 1;
+	for(std::size_t r = 0; r < p; ++r)
+		if(r + 1 != q) nfib *= na[r];
+	std::size_t counter = 0;
+	for(std::size_t f = 0; f < nfib; ++f) {
+		std::size_t rest = f, off = 0;
+		for(std::size_t s = 0; s < p; ++s) {
+			if(s == qpos) continue;
+			std::size_t const m = pia[s] - 1;
+			off += (rest % na[m]) * wa[m];
+			rest /= na[m];
+		}
+		for(std::size_t k = 0; k < na[q - 1]; ++k) a[off + k * wa[q - 1]] = static_cast<T>(++counter);
+	}
+}
+
+static std::vector<vec> shapes_of(std::size_t rank)
+{
+	std::vector<vec> out;
+	std::vector<std::size_t> idx(rank, 0);
+	for(;;) {
+		vec s(rank);
+		for(std::size_t r = 0; r < rank; ++r) s[r] = std::size_t{2} << idx[r];
+		out.push_back(s);
+		std::size_t r = 0;
+		while(r < rank && ++idx[r] == 3) idx[r++] = 0;
+		if(r == rank) break;
+	}
+	return out;
+}
+
+template<class T, class E, class S, class F>
+static void check_policy(E ep, S sp, F fp, const char* name)
+{
+	for(std::size_t rank = 2; rank <= 4; ++rank) {
+		vec perm(rank);
+		std::iota(perm.begin(), perm.end(), 1);
+		std::vector<vec> layouts;
+		do layouts.push_back(perm); while(std::next_permutation(perm.begin(), perm.end()));
+		for(auto const& na : shapes_of(rank)) {
+			std::size_t const nn = std::accumulate(na.begin(), na.end(), std::size_t{1}, std::multiplies<>());
+			std::vector<T> a(nn);
+			for(auto const& pia : layouts) {
+				auto wa = detail::generate_strides(na, pia);
+				for(std::size_t q = 1; q <= rank; ++q) {
+					known_init(q, a, na, wa, pia);
+					std::vector<T> b(na[q - 1], T{1});
+					auto pic = detail::generate_output_layout(pia, q);
+					auto nc  = detail::generate_output_shape(na, q);
+					auto wc  = detail::generate_strides(nc, pic);
+					vec nb   = {b.size()};
+					std::size_t const nnc = nn / na[q - 1];
+					std::vector<T> c(nnc, T{-1});
+					ttv(ep, sp, fp, q, rank, a.data(), na.data(), wa.data(), pia.data(), b.data(), nb.data(),
+					    c.data(), nc.data(), wc.data(), pic.data());
+					std::size_t const nq = na[q - 1];
+					bool ok = true;
+					for(std::size_t j = 0; j < nnc; ++j) {
+						T const expect = static_cast<T>((nq * nq * (2 * (j + 1) - 1) + nq) / 2);
+						ok = ok && c[j] == expect;
+					}
+					CHECK(ok, "%s rank %zu q %zu", name, rank, q);
+				}
+			}
+		}
+	}
+}
+
+template<class T>
+static void all_policies()
+{
+	namespace ep = execution_policy;
+	namespace sp = slicing_policy;
+	namespace fp = fusion_policy;
+	check_policy<T>(ep::seq, sp::subtensor, fp::none, "seq/subtensor/none");
+	check_policy<T>(ep::seq_blas, sp::subtensor, fp::none, "seq_blas/subtensor/none");
+	check_policy<T>(ep::par, sp::subtensor, fp::none, "par/subtensor/none");
+	check_policy<T>(ep::par_loop, sp::subtensor, fp::none, "par_loop/subtensor/none");
+	check_policy<T>(ep::par_loop, sp::subtensor, fp::all, "par_loop/subtensor/all");
+	check_policy<T>(ep::par_task, sp::subtensor, fp::none, "par_task/subtensor/none");
+	check_policy<T>(ep::par_taskloop, sp::subtensor, fp::none, "par_taskloop/subtensor/none");
+	check_policy<T>(ep::par_blas, sp::subtensor, fp::all, "par_blas/subtensor/all");
+	check_policy<T>(ep::seq, sp::slice, fp::none, "seq/slice/none");
+	check_policy<T>(ep::seq_blas, sp::slice, fp::none, "seq_blas/slice/none");
+	check_policy<T>(ep::par, sp::slice, fp::none, "par/slice/none");
+	check_policy<T>(ep::par_loop, sp::slice, fp::none, "par_loop/slice/none");
+	check_policy<T>(ep::par_task, sp::slice, fp::none, "par_task/slice/none");
+	check_policy<T>(ep::par_taskloop, sp::slice, fp::none, "par_taskloop/slice/none");
+	check_policy<T>(ep::par_loop, sp::slice, fp::outer, "par_loop/slice/outer");
+	check_policy<T>(ep::par_loop, sp::slice, fp::all, "par_loop/slice/all");
+	check_policy<T>(ep::par_blas, sp::slice, fp::all, "par_blas/slice/all");
+	check_policy<T>(ep::par_blas_loop, sp::slice, fp::all, "par_blas_loop/slice/all");
+}
+
+static void examples()
+{
+	// interface1.cpp: auto C1 = A(mode) * B;
+	{
+		auto A = tensor<float>({4, 3, 2});
+		auto B = tensor<float>({3, 1});
+		std::iota(A.begin(), A.end(), 1);
+		std::fill(B.begin(), B.end(), 1);
+		auto C1 = A(2) * B;
+		std::vector<float> expect = {15, 18, 21, 24, 51, 54, 57, 60};
+		CHECK(C1.data() == expect, "interface1");
+		CHECK((C1.shape() == vec{4, 2}) && (C1.layout() == vec{1, 2}), "interface1 metadata");
+	}
+	// interface2.cpp: the high-level function with two policy triples
+	{
+		auto A = tensor<float>({4, 3, 2});
+		auto B = tensor<float>({3, 1});
+		std::iota(A.begin(), A.end(), 1);
+		std::fill(B.begin(), B.end(), 1);
+		auto C1 = ttv(2, A, B, execution_policy::seq, slicing_policy::subtensor, fusion_policy::none);
+		auto C2 = ttv(2, A, B, execution_policy::par_loop, slicing_policy::slice, fusion_policy::all);
+		auto C3 = ttv(2, A, B);
+		std::vector<float> expect = {15, 18, 21, 24, 51, 54, 57, 60};
+		CHECK(C1.data() == expect && C2.data() == expect && C3.data() == expect, "interface2");
+	}
+	// interface3.cpp: the C-like low-level function on std::vector buffers
+	{
+		auto na = vec{4, 3, 2};
+		std::size_t p = na.size(), q = 2;
+		auto pia = detail::generate_k_order_layout(p, std::size_t{1});
+		auto wa  = detail::generate_strides(na, pia);
+		auto nb  = vec{na[q - 1]};
+		auto nc  = detail::generate_output_shape(na, q);
+		auto pic = detail::generate_output_layout(pia, q);
+		auto wc  = detail::generate_strides(nc, pic);
+		std::vector<float> A(24), B(3, 1.0f), C1(8, 0.0f), C2(8, 0.0f);
+		std::iota(A.begin(), A.end(), 1.0f);
+		ttv(execution_policy::seq, slicing_policy::slice, fusion_policy::none, q, p, A.data(), na.data(), wa.data(),
+		    pia.data(), B.data(), nb.data(), C1.data(), nc.data(), wc.data(), pic.data());
+		ttv(execution_policy::par_loop, slicing_policy::subtensor, fusion_policy::all, q, p, A.data(), na.data(),
+		    wa.data(), pia.data(), B.data(), nb.data(), C2.data(), nc.data(), wc.data(), pic.data());
+		std::vector<float> expect = {15, 18, 21, 24, 51, 54, 57, 60};
+		CHECK(C1 == expect && C2 == expect, "interface3");
+		CHECK(wa == (vec{1, 4, 12}) && wc == (vec{1, 4}), "interface3 strides");
+	}
+	// argument errors surface as std::runtime_error with the reference's text
+	{
+		auto A = tensor<double>({4, 3, 2});
+		auto B = tensor<double>({5, 1});
+		std::string msg;
+		try {
+			ttv(2, A, B);
+		} catch(std::runtime_error const& e) {
+			msg = e.what();
+		}
+		CHECK(msg == "Error in tlib::tensor_times_vector: contraction dimension of A and B are not equal.", "%s", msg.c_str());
+		std::string msg2;
+		try {
+			auto C = A(4) * B;
+			(void)C;
+		} catch(std::runtime_error const& e) {
+			msg2 = e.what();
+		}
+		CHECK(msg2 == "Error in tlib::tensor: specified contraction mode should be greater than one and equal to or less than the order.", "%s", msg2.c_str());
+	}
+}
+
+int main()
+{
+	examples();
+	all_policies<double>();
+	check_policy<float>(execution_policy::par_loop, slicing_policy::subtensor, fusion_policy::all, "float default");
+	std::printf("%ld checks, %ld failures\n", g_checks, g_fail);
+	return g_fail == 0 ? 0 : 1;
+}

@@ -1,0 +1,86 @@
+// Diagnostic stream kernels used by the benchmark: a pure read stream (the practical HBM read
+// ceiling the product kernels are compared with) and a plain copy (a known-size kernel that
+// calibrates the FETCH_SIZE / WRITE_SIZE counters).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/ttvx.h"
+#include "runtime.hpp"
+
+namespace ttvx
+{
+
+// Each block sums its 16-byte vectors and writes one float, so the loads cannot be dropped.
+__global__ __launch_bounds__(kBlock) void k_read_stream(const u32x4* __restrict__ src, int64_t n16, float* __restrict__ out)
+{
+	constexpr int U = 8;
+	float s = 0.f;
+	int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
+	int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	for(; i + (U - 1) * stride < n16; i += U * stride) {
+		u32x4 v[U];
+#pragma unroll
+		for(int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+		for(int u = 0; u < U; ++u) s += __uint_as_float(v[u].x) + __uint_as_float(v[u].y) + __uint_as_float(v[u].z) + __uint_as_float(v[u].w);
+	}
+	for(; i < n16; i += stride) {
+		u32x4 v = src[i];
+		s += __uint_as_float(v.x) + __uint_as_float(v.y) + __uint_as_float(v.z) + __uint_as_float(v.w);
+	}
+	for(int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+	__shared__ float ws[kBlock / 64];
+	if((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+	__syncthreads();
+	if(threadIdx.x == 0) out[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_stream(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16)
+{
+	int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
+	for(int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+} // namespace ttvx
+
+extern "C" {
+
+int ttvx_bench_read(const void* src, size_t bytes, float* partial, size_t partial_count, void* stream)
+{
+	using namespace ttvx;
+	clear_error();
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	if(bytes % 16 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_read: 16-byte aligned buffer and size required");
+	int rc = check_device_span(src, bytes, "src");
+	if(rc == TTVX_OK) rc = check_device_span(partial, partial_count * 4, "partial");
+	if(rc != TTVX_OK) return rc;
+	unsigned const grid = static_cast<unsigned>(std::min<size_t>(partial_count, static_cast<size_t>(di.cus) * 8));
+	if(grid == 0) return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_read: partial_count must be positive");
+	hipLaunchKernelGGL(k_read_stream, dim3(grid), dim3(kBlock), 0, static_cast<hipStream_t>(stream),
+	                   static_cast<const u32x4*>(src), static_cast<int64_t>(bytes / 16), partial);
+	hipError_t e = hipGetLastError();
+	return e == hipSuccess ? TTVX_OK : hip_fail(e, "read stream launch");
+}
+
+int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream)
+{
+	using namespace ttvx;
+	clear_error();
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	if(bytes % 16 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0 || reinterpret_cast<uintptr_t>(dst) % 16 != 0)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_copy: 16-byte aligned buffers and size required");
+	int rc = check_device_span(src, bytes, "src");
+	if(rc == TTVX_OK) rc = check_device_span(dst, bytes, "dst");
+	if(rc != TTVX_OK) return rc;
+	hipLaunchKernelGGL(k_copy_stream, dim3(static_cast<unsigned>(di.cus) * 8), dim3(kBlock), 0,
+	                   static_cast<hipStream_t>(stream), static_cast<const u32x4*>(src), static_cast<u32x4*>(dst),
+	                   static_cast<int64_t>(bytes / 16));
+	hipError_t e = hipGetLastError();
+	return e == hipSuccess ? TTVX_OK : hip_fail(e, "copy stream launch");
+}
+
+} // extern "C"

@@ -1,0 +1,502 @@
+// gfx950 kernels of the mode-q tensor-times-vector product.
+//
+// Every problem arrives folded to  C[o,i] = sum_k A[o*sAo + k*sAk + i] * b[k]  (plan.hpp).
+// The launch grid is the fused loop nest of the reference's Loops-over-GEMM formulation
+// (tensor_times_vector.h:1207-1256 and :705-772 fuse the free modes into one OpenMP loop and
+// call one BLAS gemv per slice): here one work item is a tile of the [O][I] output, and the
+// contraction over k runs inside the tile.  There is no transposed copy of A.
+//
+//   k_dot  (I == 1, q is the fastest mode; reference gemv_row / dot, matrix_times_vector.h:51-91,
+//           231-295): LPR lanes share a row of K contiguous elements, each lane loads 16-byte
+//           vectors, rows are reduced with cross-lane butterflies.  U rows per lane in flight.
+//   k_col  (I > 1; reference gemv_col / gemv_col_blas, matrix_times_vector.h:108-221): IT lanes
+//           along i (16-byte vectors, coalesced), KS lane groups along k, OS output slabs per
+//           item; partial sums over k meet in a butterfly (one wave) or in LDS (several waves).
+//   k_strided  general strides that do not fold to three dimensions: one thread per output.
+//   k_finalize sums the split-n_q partials (FP32 or FP64) and applies FP8 scales.
+//
+// Storage types: float, double, bfloat16 (uint16_t) and OCP FP8-e4m3 (uint8_t).  Accumulation
+// is FP64 for double storage and FP32 otherwise, with optional Kahan compensation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttvx
+{
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef float    f32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kBlock = 256;   // threads per block for every kernel below (4 waves of 64)
+
+template<typename S> struct Traits;
+template<> struct Traits<float>    { using acc = float;  static constexpr int vmax = 4;  };
+template<> struct Traits<double>   { using acc = double; static constexpr int vmax = 2;  };
+template<> struct Traits<uint16_t> { using acc = float;  static constexpr int vmax = 8;  };
+template<> struct Traits<uint8_t>  { using acc = float;  static constexpr int vmax = 16; };
+
+// ---- loads ---------------------------------------------------------------------------------
+// Loads VEC consecutive elements (VEC*sizeof(S) bytes, naturally aligned) and converts them to
+// the accumulation type.
+
+template<int BYTES> struct RawLoad;
+template<> struct RawLoad<16> {
+	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[4]) {
+		u32x4 v = *reinterpret_cast<const u32x4*>(p);
+		w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+	}
+};
+template<> struct RawLoad<8> {
+	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[2]) {
+		u32x2 v = *reinterpret_cast<const u32x2*>(p);
+		w[0] = v.x; w[1] = v.y;
+	}
+};
+template<> struct RawLoad<4> {
+	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
+		w[0] = *reinterpret_cast<const uint32_t*>(p);
+	}
+};
+template<> struct RawLoad<2> {
+	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
+		w[0] = *reinterpret_cast<const uint16_t*>(p);
+	}
+};
+template<> struct RawLoad<1> {
+	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
+		w[0] = *reinterpret_cast<const uint8_t*>(p);
+	}
+};
+
+template<typename S, int VEC> struct Vec {
+	static constexpr int bytes = VEC * static_cast<int>(sizeof(S));
+	static constexpr int words = bytes >= 4 ? bytes / 4 : 1;
+	uint32_t w[words];
+	__device__ __forceinline__ void load(const S* p) { RawLoad<bytes>::get(p, w); }
+};
+
+// decode element e of a loaded vector
+template<typename S, int VEC>
+__device__ __forceinline__ void decode(const Vec<S, VEC>& v, typename Traits<S>::acc (&x)[VEC]);
+
+template<int VEC>
+__device__ __forceinline__ void decode(const Vec<float, VEC>& v, float (&x)[VEC])
+{
+#pragma unroll
+	for(int e = 0; e < VEC; ++e) x[e] = __uint_as_float(v.w[e]);
+}
+
+template<int VEC>
+__device__ __forceinline__ void decode(const Vec<double, VEC>& v, double (&x)[VEC])
+{
+#pragma unroll
+	for(int e = 0; e < VEC; ++e)
+		x[e] = __hiloint2double(static_cast<int>(v.w[2 * e + 1]), static_cast<int>(v.w[2 * e]));
+}
+
+template<int VEC>
+__device__ __forceinline__ void decode(const Vec<uint16_t, VEC>& v, float (&x)[VEC])
+{
+	if constexpr(VEC == 1) {
+		x[0] = __uint_as_float(v.w[0] << 16);
+	} else {
+#pragma unroll
+		for(int e = 0; e < VEC / 2; ++e) {
+			x[2 * e]     = __uint_as_float(v.w[e] << 16);
+			x[2 * e + 1] = __uint_as_float(v.w[e] & 0xffff0000u);
+		}
+	}
+}
+
+// OCP e4m3 -> float with the gfx950 packed converter (two codes per instruction).
+template<int VEC>
+__device__ __forceinline__ void decode(const Vec<uint8_t, VEC>& v, float (&x)[VEC])
+{
+	if constexpr(VEC == 1) {
+		f32x2 f = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(v.w[0]), false);
+		x[0] = f.x;
+	} else if constexpr(VEC == 2) {
+		f32x2 f = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(v.w[0]), false);
+		x[0] = f.x; x[1] = f.y;
+	} else {
+#pragma unroll
+		for(int e = 0; e < VEC / 4; ++e) {
+			f32x2 lo = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(v.w[e]), false);
+			f32x2 hi = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(v.w[e]), true);
+			x[4 * e] = lo.x; x[4 * e + 1] = lo.y; x[4 * e + 2] = hi.x; x[4 * e + 3] = hi.y;
+		}
+	}
+}
+
+// ---- accumulation ----------------------------------------------------------------------------
+template<typename Acc, bool KAHAN> struct Sum;
+template<typename Acc> struct Sum<Acc, false> {
+	Acc s = Acc(0);
+	__device__ __forceinline__ void add(Acc x) { s += x; }
+	__device__ __forceinline__ void fma(Acc a, Acc b) { s = __builtin_fma(a, b, s); }
+	__device__ __forceinline__ Acc value() const { return s; }
+};
+template<typename Acc> struct Sum<Acc, true> {
+	Acc s = Acc(0), c = Acc(0);     // running sum and its compensation (Kahan 1965)
+	__device__ __forceinline__ void add(Acc x) {
+		Acc y = x - c;
+		Acc t = s + y;
+		c = (t - s) - y;
+		s = t;
+	}
+	__device__ __forceinline__ void fma(Acc a, Acc b) { add(a * b); }
+	__device__ __forceinline__ Acc value() const { return s - c; }
+};
+
+// ---- launch arguments ------------------------------------------------------------------------
+struct Args {
+	int64_t O, K, I;          // folded extents
+	int64_t sAo, sAk;         // A strides (elements); the i stride is 1 for k_dot/k_col
+	int64_t sCo;              // C stride of o (the i stride is 1)
+	// k_col tiling
+	int IT, KS, OS;           // lanes along i (vectors), lane groups along k, slabs per item
+	int red;                  // cross-lane reduction: 0 none, 1 butterfly in one wave, 2 LDS
+	int64_t iTiles;           // tiles along i
+	int64_t items;            // ceil(O / (OS*U)) * iTiles
+	// k_dot tiling
+	int LPR;                  // lanes per row (power of two <= 64)
+	int lprShift;             // log2(LPR)
+	int64_t groups;           // ceil(O / (rows per wave * U))
+	int kiter;                // vector steps per lane per row (for b caching)
+	// split of the contraction over gridDim.y
+	int64_t kchunk;           // contraction elements per split (multiple of the lane k step)
+	void*   partial;          // [gridDim.y][O*I] partial sums, or null when gridDim.y == 1
+	int     part64;           // partial sums stored as double
+	// FP8 scales (null for other types)
+	const float* scales;
+	int64_t fps;              // fibers per scale
+};
+
+template<typename Acc>
+__device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, int64_t o, int64_t i, Acc v)
+{
+	int64_t const j = o * P.I + i;
+	if(P.partial != nullptr) {
+		int64_t const off = static_cast<int64_t>(blockIdx.y) * P.O * P.I + j;
+		if(P.part64) static_cast<double*>(P.partial)[off] = static_cast<double>(v);
+		else         static_cast<Acc*>(P.partial)[off]    = v;
+		return;
+	}
+	if(P.scales != nullptr) v *= static_cast<Acc>(P.scales[j / P.fps]);
+	static_cast<Acc*>(C)[o * P.sCo + i] = v;
+}
+
+// ---- k_col --------------------------------------------------------------------------------
+// Block tile: OS*U output slabs x IT*VEC columns, all of the block's k range.
+// Thread t: it = t % IT, ks = (t / IT) % KS, os = t / (IT*KS).
+template<typename S, int VEC, int U, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                void* __restrict__ C, Args P)
+{
+	using Acc = typename Traits<S>::acc;
+	constexpr int KU = U >= 4 ? 2 : (U == 2 ? 4 : 8);   // k unroll: U*KU independent loads in flight
+	extern __shared__ unsigned char smem_raw[];
+	Acc* red = reinterpret_cast<Acc*>(smem_raw);
+
+	int const t  = static_cast<int>(threadIdx.x);
+	int const it = t % P.IT;
+	int const ks = (t / P.IT) % P.KS;
+	int const os = t / (P.IT * P.KS);
+	bool const active = os < P.OS;
+
+	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
+	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
+	int64_t const kstep = static_cast<int64_t>(P.KS) * P.sAk;
+
+	for(int64_t item = blockIdx.x; item < P.items; item += gridDim.x) {
+		int64_t const og    = item / P.iTiles;
+		int64_t const itile = item - og * P.iTiles;
+		int64_t const i     = (itile * P.IT + it) * VEC;
+		int64_t const ic    = i < P.I ? i : P.I - VEC;        // clamped: tail lanes re-read live data
+		int64_t o[U];
+		const S* pa[U];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			o[u] = (og * U + u) * P.OS + os;
+			int64_t const oc = o[u] < P.O ? o[u] : P.O - 1;
+			pa[u] = A + oc * P.sAo + (kb + ks) * P.sAk + ic;
+		}
+		Sum<Acc, KAHAN> acc[U][VEC];
+
+		if(active) {
+			int64_t k = kb + ks;
+			for(; k + (KU - 1) * P.KS < ke; k += KU * P.KS) {
+				Acc bk[KU];
+				Vec<S, VEC> raw[KU][U];
+#pragma unroll
+				for(int j = 0; j < KU; ++j) bk[j] = b[k + j * P.KS];
+#pragma unroll
+				for(int j = 0; j < KU; ++j)
+#pragma unroll
+					for(int u = 0; u < U; ++u) raw[j][u].load(pa[u] + j * kstep);
+#pragma unroll
+				for(int j = 0; j < KU; ++j)
+#pragma unroll
+					for(int u = 0; u < U; ++u) {
+						Acc x[VEC];
+						decode(raw[j][u], x);
+#pragma unroll
+						for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk[j]);
+					}
+#pragma unroll
+				for(int u = 0; u < U; ++u) pa[u] += KU * kstep;
+			}
+			for(; k < ke; k += P.KS) {
+				Acc const bk = b[k];
+#pragma unroll
+				for(int u = 0; u < U; ++u) {
+					Vec<S, VEC> raw;
+					raw.load(pa[u]);
+					Acc x[VEC];
+					decode(raw, x);
+#pragma unroll
+					for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk);
+					pa[u] += kstep;
+				}
+			}
+		}
+
+		Acc v[U][VEC];
+#pragma unroll
+		for(int u = 0; u < U; ++u)
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) v[u][e] = acc[u][e].value();
+
+		if(P.red == 1) {
+			// lanes it + IT*ks of one wave hold the same outputs
+			for(int d = P.IT; d < P.IT * P.KS; d <<= 1)
+#pragma unroll
+				for(int u = 0; u < U; ++u)
+#pragma unroll
+					for(int e = 0; e < VEC; ++e) v[u][e] += __shfl_xor(v[u][e], d, 64);
+		} else if(P.red == 2) {
+#pragma unroll
+			for(int u = 0; u < U; ++u)
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) red[(u * VEC + e) * kBlock + t] = v[u][e];
+			__syncthreads();
+			if(active && ks == 0) {
+				for(int s = 1; s < P.KS; ++s)
+#pragma unroll
+					for(int u = 0; u < U; ++u)
+#pragma unroll
+						for(int e = 0; e < VEC; ++e) v[u][e] += red[(u * VEC + e) * kBlock + t + s * P.IT];
+			}
+			__syncthreads();
+		}
+
+		if(active && ks == 0 && i < P.I) {
+#pragma unroll
+			for(int u = 0; u < U; ++u)
+				if(o[u] < P.O)
+#pragma unroll
+					for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, o[u], i + e, v[u][e]);
+		}
+	}
+}
+
+// ---- k_dot ----------------------------------------------------------------------------------
+// Wave w handles row groups g = w, w + #waves, ...; a group is U*RPW rows (RPW = 64/LPR).
+// Lane l: row r = l / LPR inside each of the U row sets, k offset (l % LPR)*VEC.
+template<typename S, int VEC, int U, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_dot(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                void* __restrict__ C, Args P)
+{
+	using Acc = typename Traits<S>::acc;
+	constexpr int BREG = 4;                       // vector steps of b cached in registers
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	int const r    = lane >> P.lprShift;
+	int const kl   = lane & (P.LPR - 1);
+	int const rpw  = 64 >> P.lprShift;
+	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
+	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
+	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
+	int64_t const wave  = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+	int64_t const waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
+
+	if(P.kiter <= BREG) {
+		// short rows: this lane's b entries stay in registers for the whole launch
+		Acc bb[BREG][VEC];
+#pragma unroll
+		for(int s = 0; s < BREG; ++s)
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) {
+				int64_t const k = k0 + s * kstride + e;
+				bb[s][e] = (s < P.kiter && k < ke) ? b[k] : Acc(0);
+			}
+		for(int64_t g = wave; g < P.groups; g += waves) {
+			int64_t row[U];
+			const S* pa[U];
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				row[u] = (g * U + u) * rpw + r;
+				int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
+				pa[u] = A + rc * P.sAo + k0;
+			}
+			Sum<Acc, KAHAN> acc[U];
+#pragma unroll
+			for(int s = 0; s < BREG; ++s) {
+				if(s < P.kiter) {
+					bool const live = k0 + s * kstride < ke;
+					Vec<S, VEC> raw[U];
+					if(live) {
+#pragma unroll
+						for(int u = 0; u < U; ++u) raw[u].load(pa[u] + s * kstride);
+#pragma unroll
+						for(int u = 0; u < U; ++u) {
+							Acc x[VEC];
+							decode(raw[u], x);
+#pragma unroll
+							for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bb[s][e]);
+						}
+					}
+				}
+			}
+			Acc v[U];
+#pragma unroll
+			for(int u = 0; u < U; ++u) v[u] = acc[u].value();
+			for(int d = 1; d < P.LPR; d <<= 1)
+#pragma unroll
+				for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
+			if(kl == 0) {
+#pragma unroll
+				for(int u = 0; u < U; ++u)
+					if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
+			}
+		}
+	} else {
+		// long rows: b is re-read through the cache, loads of U rows overlap
+		for(int64_t g = wave; g < P.groups; g += waves) {
+			int64_t row[U];
+			const S* pa[U];
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				row[u] = (g * U + u) * rpw + r;
+				int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
+				pa[u] = A + rc * P.sAo;
+			}
+			Sum<Acc, KAHAN> acc[U];
+			int64_t k = k0;
+			for(; k + kstride < ke; k += 2 * kstride) {
+				Vec<S, VEC> raw[2][U];
+				Acc bv[2][VEC];
+#pragma unroll
+				for(int h = 0; h < 2; ++h)
+#pragma unroll
+					for(int u = 0; u < U; ++u) raw[h][u].load(pa[u] + k + h * kstride);
+#pragma unroll
+				for(int h = 0; h < 2; ++h)
+#pragma unroll
+					for(int e = 0; e < VEC; ++e) bv[h][e] = b[k + h * kstride + e];
+#pragma unroll
+				for(int h = 0; h < 2; ++h)
+#pragma unroll
+					for(int u = 0; u < U; ++u) {
+						Acc x[VEC];
+						decode(raw[h][u], x);
+#pragma unroll
+						for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[h][e]);
+					}
+			}
+			if(k < ke) {
+				Acc bv[VEC];
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) bv[e] = b[k + e];
+#pragma unroll
+				for(int u = 0; u < U; ++u) {
+					Vec<S, VEC> raw;
+					raw.load(pa[u] + k);
+					Acc x[VEC];
+					decode(raw, x);
+#pragma unroll
+					for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[e]);
+				}
+			}
+			Acc v[U];
+#pragma unroll
+			for(int u = 0; u < U; ++u) v[u] = acc[u].value();
+			for(int d = 1; d < P.LPR; d <<= 1)
+#pragma unroll
+				for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
+			if(kl == 0) {
+#pragma unroll
+				for(int u = 0; u < U; ++u)
+					if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
+			}
+		}
+	}
+}
+
+// ---- k_strided ------------------------------------------------------------------------------
+constexpr int kMaxFree = 32;
+struct StridedArgs {
+	int     nd;
+	int64_t n[kMaxFree], sa[kMaxFree], sc[kMaxFree];   // free dims, fastest first
+	int64_t K, sAk, outputs;
+	const float* scales;
+	int64_t fps;
+};
+
+template<typename S, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_strided(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                    void* __restrict__ C, StridedArgs G)
+{
+	using Acc = typename Traits<S>::acc;
+	for(int64_t j = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; j < G.outputs;
+	    j += static_cast<int64_t>(gridDim.x) * kBlock) {
+		int64_t rest = j, offa = 0, offc = 0;
+		for(int d = 0; d < G.nd; ++d) {
+			int64_t const idx = rest % G.n[d];
+			rest /= G.n[d];
+			offa += idx * G.sa[d];
+			offc += idx * G.sc[d];
+		}
+		Sum<Acc, KAHAN> acc;
+		const S* pa = A + offa;
+		for(int64_t k = 0; k < G.K; ++k) {
+			Vec<S, 1> raw;
+			raw.load(pa + k * G.sAk);
+			Acc x[1];
+			decode(raw, x);
+			acc.fma(x[0], b[k]);
+		}
+		Acc v = acc.value();
+		if(G.scales != nullptr) v *= static_cast<Acc>(G.scales[j / G.fps]);
+		static_cast<Acc*>(C)[offc] = v;
+	}
+}
+
+// ---- k_finalize -----------------------------------------------------------------------------
+template<typename Acc>
+__global__ __launch_bounds__(kBlock) void k_finalize(const void* __restrict__ partial, int part64, int splits,
+                                                     int64_t O, int64_t I, int64_t sCo, void* __restrict__ C,
+                                                     const float* __restrict__ scales, int64_t fps)
+{
+	int64_t const n = O * I;
+	for(int64_t j = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; j < n;
+	    j += static_cast<int64_t>(gridDim.x) * kBlock) {
+		Acc v;
+		if(part64) {
+			double s = 0.0;
+			for(int q = 0; q < splits; ++q) s += static_cast<const double*>(partial)[q * n + j];
+			v = static_cast<Acc>(s);
+		} else {
+			Acc s = Acc(0);
+			for(int q = 0; q < splits; ++q) s += static_cast<const Acc*>(partial)[q * n + j];
+			v = s;
+		}
+		if(scales != nullptr) v *= static_cast<Acc>(scales[j / fps]);
+		int64_t const o = j / I, i = j - o * I;
+		static_cast<Acc*>(C)[o * sCo + i] = v;
+	}
+}
+
+} // namespace ttvx

@@ -1,0 +1,10 @@
+// Kernel instantiations for uint16_t storage.
+#include "launch_impl.hpp"
+
+namespace ttvx
+{
+int launch_bf16(const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s)
+{
+	return launch_typed<uint16_t>(cfg, a, b, c, kahan, s);
+}
+} // namespace ttvx

@@ -1,0 +1,73 @@
+// Compile-time dispatch of a launch configuration to the kernel instantiations of one storage
+// type.  Each launch_<type>.hip includes this file once for its type, so the four storage types
+// compile in parallel.
+#pragma once
+
+#include "runtime.hpp"
+
+namespace ttvx
+{
+
+template<typename S, int VEC, bool KH>
+int launch_vec(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
+{
+	dim3 const grid(static_cast<unsigned>(cfg.grid), static_cast<unsigned>(cfg.ksplit));
+	dim3 const block(kBlock);
+	if(cfg.kind == KIND_DOT) {
+		switch(cfg.U) {
+			case 1: hipLaunchKernelGGL((k_dot<S, VEC, 1, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
+			case 2: hipLaunchKernelGGL((k_dot<S, VEC, 2, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
+			case 4: hipLaunchKernelGGL((k_dot<S, VEC, 4, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
+			case 8: hipLaunchKernelGGL((k_dot<S, VEC, 8, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
+			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported row unroll");
+		}
+	} else {
+		switch(cfg.U) {
+			case 1: hipLaunchKernelGGL((k_col<S, VEC, 1, KH>), grid, block, cfg.shmem, s, A, B, C, cfg.args); break;
+			case 2: hipLaunchKernelGGL((k_col<S, VEC, 2, KH>), grid, block, cfg.shmem, s, A, B, C, cfg.args); break;
+			case 4: hipLaunchKernelGGL((k_col<S, VEC, 4, KH>), grid, block, cfg.shmem, s, A, B, C, cfg.args); break;
+			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported slab unroll");
+		}
+	}
+	return TTVX_OK;
+}
+
+// Tries the vector widths V, V/2, ..., 1 (all instantiated).
+template<typename S, bool KH, int V>
+int launch_width(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
+{
+	if(cfg.vec == V) return launch_vec<S, V, KH>(cfg, A, B, C, s);
+	if constexpr(V > 1) return launch_width<S, KH, V / 2>(cfg, A, B, C, s);
+	else return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported vector width");
+}
+
+template<typename S, bool KH>
+int launch_kh(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
+{
+	if(cfg.kind == KIND_STRIDED) {
+		hipLaunchKernelGGL((k_strided<S, KH>), dim3(static_cast<unsigned>(cfg.grid)), dim3(kBlock), 0, s, A, B, C, cfg.sargs);
+		return TTVX_OK;
+	}
+	return launch_width<S, KH, Traits<S>::vmax>(cfg, A, B, C, s);
+}
+
+template<typename S>
+int launch_typed(const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s)
+{
+	using Acc = typename Traits<S>::acc;
+	auto const* A = static_cast<const S*>(a);
+	auto const* B = static_cast<const Acc*>(b);
+	int rc;
+	if constexpr(std::is_same_v<Acc, double>) {
+		(void)kahan;   // FP64 accumulation needs no compensation
+		rc = launch_kh<S, false>(cfg, A, B, c, s);
+	} else {
+		rc = kahan ? launch_kh<S, true>(cfg, A, B, c, s) : launch_kh<S, false>(cfg, A, B, c, s);
+	}
+	if(rc != TTVX_OK) return rc;
+	hipError_t const e = hipGetLastError();
+	if(e != hipSuccess) return hip_fail(e, "kernel launch");
+	return TTVX_OK;
+}
+
+} // namespace ttvx

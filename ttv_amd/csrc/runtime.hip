@@ -1,0 +1,400 @@
+// Host runtime of libttvx: error state, device checks, variant catalogue, launch configuration,
+// fixed selection rule and the launch itself.
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/ttvx.h"
+
+namespace ttvx
+{
+
+// ---- error state --------------------------------------------------------------------------
+namespace
+{
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+int fail(int status, const std::string& msg)
+{
+	set_error(msg);
+	return status;
+}
+int hip_fail(hipError_t e, const char* where)
+{
+	std::string m = "ttvx: HIP error in ";
+	m += where;
+	m += ": ";
+	m += hipGetErrorString(e);
+	return fail(TTVX_ERR_DEVICE, m);
+}
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+// ---- device --------------------------------------------------------------------------------
+namespace
+{
+std::mutex g_dev_mutex;
+std::vector<DeviceInfo> g_devices;   // cached per device id
+} // namespace
+
+bool current_device(DeviceInfo& info)
+{
+	int id = 0;
+	hipError_t e = hipGetDevice(&id);
+	if(e != hipSuccess) {
+		(void)hipGetLastError();
+		set_error("ttvx: no HIP device available");
+		return false;
+	}
+	std::lock_guard<std::mutex> lock(g_dev_mutex);
+	if(static_cast<int>(g_devices.size()) <= id) g_devices.resize(id + 1);
+	DeviceInfo& d = g_devices[id];
+	if(d.id < 0) {
+		hipDeviceProp_t prop;
+		e = hipGetDeviceProperties(&prop, id);
+		if(e != hipSuccess) {
+			(void)hipGetLastError();
+			set_error("ttvx: hipGetDeviceProperties failed");
+			return false;
+		}
+		d.id     = id;
+		d.cus    = prop.multiProcessorCount;
+		d.arch   = prop.gcnArchName;
+		d.gfx950 = d.arch.rfind("gfx950", 0) == 0;
+	}
+	if(!d.gfx950) {
+		set_error("ttvx: device " + std::to_string(id) + " is " + d.arch + "; this library is built for gfx950 (MI355X)");
+		return false;
+	}
+	info = d;
+	return true;
+}
+
+int check_device_span(const void* p, std::size_t bytes, const char* what)
+{
+	hipDeviceptr_t base = nullptr;
+	std::size_t size = 0;
+	hipError_t e = hipMemGetAddressRange(&base, &size, const_cast<void*>(p));
+	if(e != hipSuccess || base == nullptr) {
+		(void)hipGetLastError();
+		return fail(TTVX_ERR_ARGUMENT, std::string("ttvx: ") + what + " is not device memory (stream-ordered entry points take device pointers)");
+	}
+	auto const lo = reinterpret_cast<uintptr_t>(base), at = reinterpret_cast<uintptr_t>(p);
+	if(at < lo || at + bytes > lo + size)
+		return fail(TTVX_ERR_ARGUMENT, std::string("ttvx: ") + what + " spans past the end of its device allocation");
+	return TTVX_OK;
+}
+
+// ---- variants --------------------------------------------------------------------------------
+namespace
+{
+const char* const kVariantNames[V_COUNT] = {
+	"dot_u1", "dot_u2", "dot_u4", "dot_u8", "dot_split",
+	"col_u1", "col_u2", "col_u4", "col_narrow", "col_split",
+	"col_k_lanes", "strided",
+};
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+int64_t pow2_ceil(int64_t x)
+{
+	int64_t p = 1;
+	while(p < x) p <<= 1;
+	return p;
+}
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+int ilog2(int64_t x)
+{
+	int r = 0;
+	while((int64_t{1} << (r + 1)) <= x) ++r;
+	return r;
+}
+} // namespace
+
+const char* variant_name(int v) { return (v >= 0 && v < V_COUNT) ? kVariantNames[v] : ""; }
+
+DType dtype_of(int code)
+{
+	switch(code) {
+		case TTVX_F32:     return {TTVX_F32, 4, 4};
+		case TTVX_F64:     return {TTVX_F64, 8, 2};
+		case TTVX_BF16:    return {TTVX_BF16, 2, 8};
+		case TTVX_FP8E4M3: return {TTVX_FP8E4M3, 1, 16};
+		default:           return {-1, 0, 0};
+	}
+}
+
+namespace
+{
+// Widest power-of-two vector (elements) whose loads stay aligned for every index the kernel visits.
+int pick_vec(const DType& dt, uintptr_t a_addr, int64_t inner, std::initializer_list<int64_t> strides)
+{
+	int v = dt.vmax;
+	while(v > 1) {
+		bool ok = inner % v == 0 && a_addr % static_cast<uintptr_t>(v * dt.size) == 0;
+		for(int64_t s : strides) ok = ok && s % v == 0;
+		if(ok) break;
+		v >>= 1;
+	}
+	return v;
+}
+
+std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
+
+constexpr int kGridPerCU = 16;   // resident-block cap of the grid-stride loops
+} // namespace
+
+bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus, unsigned flags, Config& cfg)
+{
+	cfg = Config{};
+	cfg.variant = v;
+	Args& P = cfg.args;
+	std::memset(&P, 0, sizeof(P));
+	P.fps = 1;
+	bool const fp64_final = (flags & TTVX_FLAG_FP64_FINAL) != 0 || f.K > (int64_t{1} << 20);
+	int64_t const gridcap = static_cast<int64_t>(std::max(cus, 1)) * kGridPerCU;
+
+	if(v == V_STRIDED) {
+		if(static_cast<int>(f.dims.size()) > kMaxFree) return false;
+		cfg.kind = KIND_STRIDED;
+		StridedArgs& G = cfg.sargs;
+		std::memset(&G, 0, sizeof(G));
+		G.nd = static_cast<int>(f.dims.size());
+		for(int d = 0; d < G.nd; ++d) { G.n[d] = f.dims[d].n; G.sa[d] = f.dims[d].sa; G.sc[d] = f.dims[d].sc; }
+		G.K = f.K; G.sAk = f.sAk; G.outputs = f.outputs; G.fps = 1;
+		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(f.outputs, kBlock), gridcap)));
+		return true;
+	}
+	if(!f.three_d) return false;
+
+	P.O = f.O; P.K = f.K; P.I = f.I;
+	P.sAo = f.sAo; P.sAk = f.sAk; P.sCo = f.sCo;
+
+	bool const dot_shape = f.I == 1 && (f.sAk == 1 || f.K == 1);
+	bool const col_shape = f.I > 1 && f.sAi == 1 && f.sCi == 1;
+
+	if(v >= V_DOT_U1 && v <= V_DOT_SPLIT) {
+		if(!dot_shape) return false;
+		cfg.kind = KIND_DOT;
+		cfg.vec  = pick_vec(dt, a_addr, f.K, {f.O > 1 ? f.sAo : 0});
+		int64_t const vsteps = ceil_div(f.K, cfg.vec);
+		P.LPR = static_cast<int>(std::min<int64_t>(64, pow2_ceil(vsteps)));
+		P.lprShift = ilog2(P.LPR);
+		int64_t const kstride = static_cast<int64_t>(P.LPR) * cfg.vec;
+		static const int us[] = {1, 2, 4, 8, 2};
+		cfg.U = us[v - V_DOT_U1];
+		int64_t const rpw = 64 / P.LPR;
+		P.groups = ceil_div(f.O, rpw * cfg.U);
+		int64_t const waves_wanted = static_cast<int64_t>(std::max(cus, 1)) * 4 * 4;
+		int64_t ksplit = 1;
+		if(v == V_DOT_SPLIT) {
+			if(P.groups >= waves_wanted || vsteps < 4 * P.LPR) return false;  // nothing to gain
+			ksplit = std::min<int64_t>(64, ceil_div(waves_wanted, P.groups));
+		}
+		int64_t kchunk = ceil_div(ceil_div(f.K, ksplit), kstride) * kstride;
+		ksplit = ceil_div(f.K, kchunk);
+		P.kchunk = kchunk;
+		P.kiter  = static_cast<int>(std::min<int64_t>(1 << 30, ceil_div(std::min(kchunk, f.K), kstride)));
+		cfg.ksplit = static_cast<int>(ksplit);
+		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, kBlock / 64), gridcap)));
+	} else if(v >= V_COL_U1 && v <= V_COL_SPLIT) {
+		if(!col_shape) return false;
+		cfg.kind = KIND_COL;
+		cfg.vec  = pick_vec(dt, a_addr, f.I, {f.K > 1 ? f.sAk : 0, f.O > 1 ? f.sAo : 0});
+		int64_t const iv = f.I / cfg.vec;
+		int64_t const itmax = v == V_COL_NARROW ? 64 : kBlock;
+		P.IT = static_cast<int>(std::min<int64_t>(iv, itmax));
+		P.KS = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kBlock / P.IT, f.K)));
+		P.OS = std::max(1, kBlock / (P.IT * P.KS));
+		static const int us[] = {1, 2, 4, 1, 1};
+		cfg.U = us[v - V_COL_U1];
+		P.iTiles = ceil_div(iv, P.IT);
+		P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS) * cfg.U) * P.iTiles;
+		int64_t ksplit = 1;
+		if(v == V_COL_SPLIT) {
+			int64_t const want = static_cast<int64_t>(std::max(cus, 1)) * 8;
+			if(P.items >= want || f.K / P.KS < 16) return false;
+			ksplit = std::min<int64_t>(64, std::min<int64_t>(ceil_div(want, P.items), f.K / P.KS / 8));
+			if(ksplit < 2) return false;
+		}
+		int64_t const kchunk = ceil_div(f.K, ksplit);
+		P.kchunk = kchunk;
+		cfg.ksplit = static_cast<int>(ceil_div(f.K, kchunk));
+		cfg.grid   = static_cast<int>(std::max<int64_t>(1, std::min(P.items, gridcap)));
+	} else if(v == V_COL_K_LANES) {
+		if(!dot_shape) return false;
+		// the column kernel on a row problem: lanes along k (scalar loads), butterfly over 64 lanes
+		cfg.kind = KIND_COL;
+		cfg.vec  = 1;
+		P.I = 1; P.IT = 1;
+		P.KS = static_cast<int>(std::min<int64_t>(64, pow2_ceil(f.K)));
+		P.OS = kBlock / P.KS;
+		P.sAk = 1;
+		cfg.U = 1;
+		P.iTiles = 1;
+		P.items  = ceil_div(f.O, P.OS);
+		P.kchunk = f.K;
+		cfg.ksplit = 1;
+		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(P.items, gridcap)));
+	} else {
+		return false;
+	}
+
+	if(cfg.kind == KIND_COL) {
+		if(P.KS == 1) P.red = 0;
+		else if(is_pow2(P.IT) && is_pow2(P.KS) && P.IT * P.KS <= 64) P.red = 1;
+		else P.red = 2;
+		cfg.shmem = P.red == 2 ? static_cast<std::size_t>(cfg.U) * cfg.vec * kBlock * acc_size(dt) : 0;
+		if(cfg.shmem > 64 * 1024) return false;
+	}
+	if(fp64_final && dt.code != TTVX_F64) P.part64 = 1;
+	return true;
+}
+
+int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags)
+{
+	Config cfg;
+	int const cus = 256;
+	if(!f.three_d) return V_STRIDED;
+	if(f.I == 1 && (f.sAk == 1 || f.K == 1)) {
+		// rows of K elements: enough rows in flight per lane, split long rows when rows are few
+		if(configure(V_DOT_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_DOT_SPLIT;
+		configure(V_DOT_U1, f, dt, a_addr, cus, flags, cfg);
+		int64_t const kiter = cfg.args.kiter;
+		if(kiter <= 1) return V_DOT_U8;
+		if(kiter <= 2) return V_DOT_U4;
+		if(kiter <= 4) return V_DOT_U2;
+		return V_DOT_U2;
+	}
+	if(f.I > 1 && f.sAi == 1 && f.sCi == 1) {
+		if(configure(V_COL_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_COL_SPLIT;
+		configure(V_COL_U1, f, dt, a_addr, cus, flags, cfg);
+		int64_t const per_thread = f.K / std::max(1, cfg.args.KS);
+		if(per_thread >= 8) return V_COL_U1;
+		if(per_thread >= 4) return V_COL_U2;
+		return V_COL_U4;
+	}
+	return V_STRIDED;
+}
+
+namespace
+{
+std::atomic<int> g_sel_mode{TTVX_SELECT_HEURISTIC};
+std::atomic<int> g_forced{0};
+} // namespace
+
+int set_selection(int mode, int forced)
+{
+	if(mode != TTVX_SELECT_HEURISTIC && mode != TTVX_SELECT_LEARNED && mode != TTVX_SELECT_FORCED)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx: unknown selection mode");
+	if(mode == TTVX_SELECT_FORCED && (forced < 0 || forced >= V_COUNT))
+		return fail(TTVX_ERR_ARGUMENT, "ttvx: forced variant out of range");
+	g_forced.store(forced);
+	g_sel_mode.store(mode);
+	return TTVX_OK;
+}
+
+int select_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags)
+{
+	int const mode = g_sel_mode.load();
+	if(mode == TTVX_SELECT_FORCED) {
+		Config cfg;
+		int const v = g_forced.load();
+		if(configure(v, f, dt, a_addr, 256, flags, cfg)) return v;
+	} else if(mode == TTVX_SELECT_LEARNED) {
+		int const v = learned_variant(f, dt, a_addr, flags);
+		if(v >= 0) return v;
+	}
+	return heuristic_variant(f, dt, a_addr, flags);
+}
+
+void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
+{
+	for(int i = 0; i < TTVX_NUM_FEATURES; ++i) out[i] = 0.f;
+	auto lg = [](int64_t x) { return x > 0 ? static_cast<float>(std::log2(static_cast<double>(x))) : 0.f; };
+	int64_t N = 1;
+	for(auto n : f.n_layout) N *= n;
+	out[0] = static_cast<float>(f.p);
+	out[1] = f.p > 1 ? static_cast<float>(f.qpos) / static_cast<float>(f.p - 1) : 0.f;
+	out[2] = lg(f.I);
+	out[3] = lg(f.K);
+	out[4] = lg(f.O);
+	out[5] = lg(N);
+	out[6] = dt.code == TTVX_F32 ? 1.f : 0.f;
+	out[7] = dt.code == TTVX_BF16 ? 1.f : 0.f;
+	out[8] = dt.code == TTVX_FP8E4M3 ? 1.f : 0.f;
+	out[9] = dt.code == TTVX_F64 ? 1.f : 0.f;
+	out[10] = f.I == 1 ? 1.f : 0.f;
+	out[11] = static_cast<float>(pick_vec(dt, a_addr, f.I > 1 ? f.I : f.K, {})) / static_cast<float>(dt.vmax);
+	out[12] = f.three_d ? 1.f : 0.f;
+	for(int s = 0; s < 10 && s < static_cast<int>(f.n_layout.size()); ++s) out[13 + s] = lg(f.n_layout[s]);
+	out[23] = static_cast<float>(f.qpos);
+}
+
+// ---- launch ----------------------------------------------------------------------------------
+int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c,
+           const float* scales, int64_t fps, unsigned flags, int variant, hipStream_t stream)
+{
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	uintptr_t const a_addr = reinterpret_cast<uintptr_t>(a);
+	if(variant < 0) variant = select_variant(f, dt, a_addr, flags);
+	Config cfg;
+	if(!configure(variant, f, dt, a_addr, di.cus, flags, cfg)) {
+		variant = heuristic_variant(f, dt, a_addr, flags);
+		if(!configure(variant, f, dt, a_addr, di.cus, flags, cfg))
+			return fail(TTVX_ERR_UNSUPPORTED, "ttvx: no kernel variant can run this problem");
+	}
+	bool const kahan = (flags & TTVX_FLAG_KAHAN) != 0;
+
+	if(cfg.kind == KIND_STRIDED) {
+		cfg.sargs.scales = scales;
+		cfg.sargs.fps    = fps > 0 ? fps : 1;
+	}
+	bool const use_partial = cfg.kind != KIND_STRIDED && (cfg.ksplit > 1 || cfg.args.part64);
+	void* partial = nullptr;
+	if(use_partial) {
+		std::size_t const esz = cfg.args.part64 ? 8 : acc_size(dt);
+		std::size_t const bytes = static_cast<std::size_t>(cfg.ksplit) * static_cast<std::size_t>(f.O * f.I) * esz;
+		hipError_t e = hipMallocAsync(&partial, bytes, stream);
+		if(e != hipSuccess) return hip_fail(e, "hipMallocAsync (split partials)");
+		cfg.args.partial = partial;
+	} else if(cfg.kind != KIND_STRIDED) {
+		cfg.args.scales = scales;
+		cfg.args.fps    = fps > 0 ? fps : 1;
+	}
+
+	int rc = TTVX_OK;
+	switch(dt.code) {
+		case TTVX_F32:     rc = launch_f32 (cfg, a, b, c, kahan, stream); break;
+		case TTVX_F64:     rc = launch_f64 (cfg, a, b, c, kahan, stream); break;
+		case TTVX_BF16:    rc = launch_bf16(cfg, a, b, c, kahan, stream); break;
+		case TTVX_FP8E4M3: rc = launch_fp8 (cfg, a, b, c, kahan, stream); break;
+		default:           rc = fail(TTVX_ERR_UNSUPPORTED, "ttvx: unknown element type");
+	}
+	if(rc == TTVX_OK && use_partial) {
+		int64_t const n = f.O * f.I;
+		unsigned const grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock), 4096)));
+		if(dt.code == TTVX_F64)
+			hipLaunchKernelGGL((k_finalize<double>), dim3(grid), dim3(kBlock), 0, stream, partial, cfg.args.part64,
+			                   cfg.ksplit, f.O, f.I, f.sCo, c, scales, fps > 0 ? fps : 1);
+		else
+			hipLaunchKernelGGL((k_finalize<float>), dim3(grid), dim3(kBlock), 0, stream, partial, cfg.args.part64,
+			                   cfg.ksplit, f.O, f.I, f.sCo, c, scales, fps > 0 ? fps : 1);
+		hipError_t e = hipGetLastError();
+		if(e != hipSuccess) rc = hip_fail(e, "finalize launch");
+	}
+	if(partial != nullptr) {
+		hipError_t e = hipFreeAsync(partial, stream);
+		if(e != hipSuccess && rc == TTVX_OK) rc = hip_fail(e, "hipFreeAsync (split partials)");
+	}
+	return rc;
+}
+
+} // namespace ttvx

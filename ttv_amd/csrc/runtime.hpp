@@ -1,0 +1,98 @@
+// Shared host-side runtime of libttvx: error state, device properties, variant catalogue and
+// launch configuration.  Used by the plain, quantised, batched and selector entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/ttvx.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace ttvx
+{
+
+// ---- error state --------------------------------------------------------------------------
+void set_error(const std::string& msg);
+void clear_error();
+int  fail(int status, const std::string& msg);          // set_error + return status
+int  hip_fail(hipError_t e, const char* where);         // TTVX_ERR_DEVICE with HIP's text
+
+// ---- device --------------------------------------------------------------------------------
+struct DeviceInfo {
+	int id = -1;
+	int cus = 0;
+	bool gfx950 = false;
+	std::string arch;
+};
+// Properties of the current device; returns false (and sets the error) without a usable GPU.
+bool current_device(DeviceInfo& info);
+
+// Checks that [p, p + bytes) lies inside one device allocation (guards against host pointers
+// and out-of-bounds spans reaching a kernel).
+int check_device_span(const void* p, std::size_t bytes, const char* what);
+
+// ---- variants --------------------------------------------------------------------------------
+enum Kind { KIND_DOT = 0, KIND_COL = 1, KIND_STRIDED = 2 };
+
+enum Variant {
+	V_DOT_U1 = 0, V_DOT_U2, V_DOT_U4, V_DOT_U8, V_DOT_SPLIT,
+	V_COL_U1, V_COL_U2, V_COL_U4, V_COL_NARROW, V_COL_SPLIT,
+	V_COL_K_LANES, V_STRIDED,
+	V_COUNT
+};
+const char* variant_name(int v);
+
+// Element type code (TTVX_F32 ...) -> storage properties
+struct DType {
+	int code;
+	int size;     // bytes per stored element
+	int vmax;     // elements per 16-byte vector
+};
+DType dtype_of(int code);
+
+struct Config {
+	Kind kind = KIND_STRIDED;
+	int vec = 1;        // elements per load
+	int U = 1;
+	int variant = V_STRIDED;
+	Args args{};
+	StridedArgs sargs{};
+	int ksplit = 1;
+	int grid = 1;
+	std::size_t shmem = 0;
+};
+
+// Builds the launch configuration of variant v; false when v cannot run this problem.
+bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus, unsigned flags, Config& cfg);
+
+// Variant chosen by the fixed rule.
+int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags);
+
+// Variant chosen by the process-wide selection mode (heuristic / learned / forced).
+int select_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags);
+
+// Selector features of a folded problem (TTVX_NUM_FEATURES floats).
+void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out);
+
+// Learned selector: argmin of the MLP's predicted relative time over valid variants, or -1
+// when no weights are loaded.
+int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags);
+
+// Loads/clears the learned selector's weights (layout in mlp.hpp); count 0 clears.
+int set_selector_weights(const float* params, std::size_t count);
+
+// Typed launchers, one per storage type (launch_<type>.hip).
+int launch_f32 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
+int launch_f64 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
+int launch_bf16(const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
+int launch_fp8 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
+
+// Enqueues C = A x_q b for a folded problem with device pointers.
+int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c,
+           const float* scales, int64_t fps, unsigned flags, int variant, hipStream_t stream);
+
+} // namespace ttvx

@@ -1,0 +1,98 @@
+// Learned selection at plan time: host inference of the selector MLP (mlp.hpp) with the
+// weights set through ttvx_selector_set_weights, plus a per-shape cache of decisions.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ttvx.h"
+#include "mlp.hpp"
+#include "runtime.hpp"
+
+namespace ttvx
+{
+
+namespace
+{
+std::mutex g_sel_mutex;
+std::shared_ptr<const std::vector<float>> g_weights;
+// decision cache: (dtype, O, K, I, alignment class, qpos, p) -> variant
+using Key = std::tuple<int, int64_t, int64_t, int64_t, int, int, int>;
+std::map<Key, int> g_cache;
+} // namespace
+
+void mlp_forward_host(const float* w, const float* x, float* y)
+{
+	float xs[kMlpF], h1[kMlpH], h2[kMlpH];
+	for(int i = 0; i < kMlpF; ++i) xs[i] = (x[i] - w[kMlpOffMean + i]) * w[kMlpOffStd + i];
+	for(int h = 0; h < kMlpH; ++h) {
+		float s = w[kMlpOffB1 + h];
+		for(int i = 0; i < kMlpF; ++i) s += w[kMlpOffW1 + static_cast<std::size_t>(h) * kMlpF + i] * xs[i];
+		h1[h] = s > 0.f ? s : 0.f;
+	}
+	for(int h = 0; h < kMlpH; ++h) {
+		float s = w[kMlpOffB2 + h];
+		for(int i = 0; i < kMlpH; ++i) s += w[kMlpOffW2 + static_cast<std::size_t>(h) * kMlpH + i] * h1[i];
+		h2[h] = s > 0.f ? s : 0.f;
+	}
+	for(int v = 0; v < kMlpV; ++v) {
+		float s = w[kMlpOffB3 + v];
+		for(int i = 0; i < kMlpH; ++i) s += w[kMlpOffW3 + static_cast<std::size_t>(v) * kMlpH + i] * h2[i];
+		y[v] = s;
+	}
+}
+
+int set_selector_weights(const float* params, std::size_t count)
+{
+	std::lock_guard<std::mutex> lock(g_sel_mutex);
+	g_cache.clear();
+	if(count == 0) {
+		g_weights.reset();
+		return TTVX_OK;
+	}
+	if(params == nullptr || count != kMlpParams)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx: selector weights must hold " + std::to_string(kMlpParams) + " floats");
+	g_weights = std::make_shared<const std::vector<float>>(params, params + count);
+	return TTVX_OK;
+}
+
+int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags)
+{
+	std::shared_ptr<const std::vector<float>> w;
+	int const align = static_cast<int>(a_addr % 16);
+	Key const key{dt.code, f.O, f.K, f.I, align, f.qpos, f.p};
+	{
+		std::lock_guard<std::mutex> lock(g_sel_mutex);
+		w = g_weights;
+		if(!w) return -1;
+		auto it = g_cache.find(key);
+		if(it != g_cache.end()) return it->second;
+	}
+	float x[kMlpF], y[kMlpV];
+	features_of(f, dt, a_addr, x);
+	mlp_forward_host(w->data(), x, y);
+	int best = -1;
+	Config cfg;
+	for(int v = 0; v < kMlpV; ++v) {
+		if(!configure(v, f, dt, a_addr, 256, flags, cfg)) continue;
+		if(best < 0 || y[v] < y[best]) best = v;
+	}
+	std::lock_guard<std::mutex> lock(g_sel_mutex);
+	if(g_cache.size() > 4096) g_cache.clear();
+	g_cache[key] = best;
+	return best;
+}
+
+} // namespace ttvx
+
+extern "C" {
+int ttvx_selector_set_weights(const float* params, size_t count)
+{
+	ttvx::clear_error();
+	return ttvx::set_selector_weights(params, count);
+}
+size_t ttvx_selector_num_params(void) { return ttvx::kMlpParams; }
+}
