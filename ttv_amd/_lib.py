@@ -59,6 +59,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libttvx.so is not built; run `python -m ttv_amd.build` (no CPU fallback exists)")
+        # torch ships its own HIP runtime with the same SONAME (libamdhip64.so.7); loading torch
+        # first makes libttvx bind to that one, so both share one runtime and one set of pointers.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             f = getattr(L, name, None)
