@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--profile", action="store_true",
+                    help="short fixed dispatch sequence for rocprofv3: copy and read streams, then each of the "
+                         "8 products 3 times; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are out of scope (the mode-q product "
@@ -93,6 +96,23 @@ def main():
 
     def run_case(c):
         ttv_amd.ttv(c["q"], c["A"], c["b"], out=c["C"])
+
+    if args.profile:
+        L = ttv_amd.lib()
+        dst = torch.empty_like(Aflat)
+        partial = torch.empty(256 * 8, dtype=torch.float32, device=dev)
+        for _ in range(3):
+            ttv_amd.check(L.ttvx_bench_copy(Aflat.data_ptr(), dst.data_ptr(), N * 4, stream.cuda_stream))
+        for _ in range(3):
+            ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
+                                            stream.cuda_stream))
+        for c in cases:
+            log("profile case", c["layout"], c["q"], c["kernel"])
+            for _ in range(3):
+                run_case(c)
+        torch.cuda.synchronize()
+        log("profile sequence done")
+        return
 
     for _ in range(args.warmup):
         for c in cases:

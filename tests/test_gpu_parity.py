@@ -237,11 +237,12 @@ def test_argument_errors_on_device(ttvx, device):
         ttvx.ttv(1, torch.zeros((), device=device), torch.zeros(1, device=device))
     with pytest.raises(ttvx.TTVError, match="shape vector of C is not valid"):
         ttvx.ttv(1, torch.zeros(3, device=device), torch.zeros(3, device=device))
-    # a span past the end of the allocation is refused before any kernel runs
+    # host memory is refused by the stream-ordered entry point before any kernel runs
     import ctypes
     from ttv_amd._lib import sizes
-    small = torch.zeros(10, device=device)
-    rc = ttvx.lib().ttvx_ttv_f32_async(1, 2, ctypes.c_void_p(small.data_ptr()), sizes([4, 4]), sizes([1, 4]),
-                                       sizes([1, 2]), ctypes.c_void_p(small.data_ptr()), sizes([4]),
-                                       ctypes.c_void_p(small.data_ptr()), sizes([4]), sizes([1]), sizes([1]), 0, None)
-    assert rc == 1 and "past the end" in ttvx.lib().ttvx_last_error().decode()
+    host = np.zeros(16, np.float32)
+    dev_buf = torch.zeros(16, device=device)
+    rc = ttvx.lib().ttvx_ttv_f32_async(1, 2, ctypes.c_void_p(host.ctypes.data), sizes([4, 4]), sizes([1, 4]),
+                                       sizes([1, 2]), ctypes.c_void_p(dev_buf.data_ptr()), sizes([4]),
+                                       ctypes.c_void_p(dev_buf.data_ptr()), sizes([4]), sizes([1]), sizes([1]), 0, None)
+    assert rc == 1 and "not device memory" in ttvx.lib().ttvx_last_error().decode()
