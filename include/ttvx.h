@@ -170,6 +170,30 @@ int ttvx_features(int dtype, size_t q, size_t p, const size_t* na, const size_t*
 size_t ttvx_selector_num_params(void);
 int ttvx_selector_set_weights(const float* params, size_t count);
 
+/* ---- batched products (no counterpart in the reference: it runs one product per call) ---- */
+/* One FP32 problem of a batch; C's extents and layout are derived from A as in the reference
+ * (shape.h:103-158, layout.h:143-207); wc = NULL means C's dense strides in that layout. */
+typedef struct ttvx_batch_problem {
+	size_t q, p;
+	const float* a;
+	const size_t* na;
+	const size_t* wa;
+	const size_t* pia;
+	const float* b;
+	float* c;
+	const size_t* wc;
+} ttvx_batch_problem;
+
+/* Validates and plans `count` problems (device pointers, fixed for the batch's lifetime) and
+ * returns an opaque handle. */
+int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** batch);
+/* Computes every problem of the batch in one persistent launch on `stream`; with use_graph != 0
+ * the launch is captured into a hipGraph on first use and replayed afterwards. */
+int ttvx_batch_launch(void* batch, int use_graph, void* stream);
+/* out4 = {tiles, algorithmic bytes, persistent grid size, problems}. */
+int ttvx_batch_info(void* batch, int64_t* out4);
+int ttvx_batch_destroy(void* batch);
+
 /* ---- diagnostics (no counterpart in the reference; used by the benchmark) -------------- */
 /* Pure read stream over `bytes` (16-byte aligned) writing one float per block to `partial`
  * (partial_count >= number of blocks, at most 8 per CU): the practical HBM read ceiling. */

@@ -188,250 +188,284 @@ __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, i
 	static_cast<Acc*>(C)[o * P.sCo + i] = v;
 }
 
-// ---- k_col --------------------------------------------------------------------------------
+// ---- column tiles -------------------------------------------------------------------------
 // Block tile: OS*U output slabs x IT*VEC columns, all of the block's k range.
-// Thread t: it = t % IT, ks = (t / IT) % KS, os = t / (IT*KS).
+// Thread t: it = t % IT, ks = (t / IT) % KS, os = t / (IT*KS).  Every thread of the block calls
+// col_item for the same item (the LDS reduction synchronises the block).
 template<typename S, int VEC, int U, bool KAHAN>
-__global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
-                                                void* __restrict__ C, Args P)
+__device__ __forceinline__ void col_item(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                         void* __restrict__ C, const Args& P, int64_t item, int64_t kb, int64_t ke,
+                                         typename Traits<S>::acc* red)
 {
 	using Acc = typename Traits<S>::acc;
 	constexpr int KU = U >= 4 ? 2 : (U == 2 ? 4 : 8);   // k unroll: U*KU independent loads in flight
-	extern __shared__ unsigned char smem_raw[];
-	Acc* red = reinterpret_cast<Acc*>(smem_raw);
-
 	int const t  = static_cast<int>(threadIdx.x);
 	int const it = t % P.IT;
 	int const ks = (t / P.IT) % P.KS;
 	int const os = t / (P.IT * P.KS);
 	bool const active = os < P.OS;
-
-	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
-	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
 	int64_t const kstep = static_cast<int64_t>(P.KS) * P.sAk;
 
-	for(int64_t item = blockIdx.x; item < P.items; item += gridDim.x) {
-		int64_t const og    = item / P.iTiles;
-		int64_t const itile = item - og * P.iTiles;
-		int64_t const i     = (itile * P.IT + it) * VEC;
-		int64_t const ic    = i < P.I ? i : P.I - VEC;        // clamped: tail lanes re-read live data
-		int64_t o[U];
-		const S* pa[U];
+	int64_t const og    = item / P.iTiles;
+	int64_t const itile = item - og * P.iTiles;
+	int64_t const i     = (itile * P.IT + it) * VEC;
+	int64_t const ic    = i < P.I ? i : P.I - VEC;        // clamped: tail lanes re-read live data
+	int64_t o[U];
+	const S* pa[U];
 #pragma unroll
-		for(int u = 0; u < U; ++u) {
-			o[u] = (og * U + u) * P.OS + os;
-			int64_t const oc = o[u] < P.O ? o[u] : P.O - 1;
-			pa[u] = A + oc * P.sAo + (kb + ks) * P.sAk + ic;
-		}
-		Sum<Acc, KAHAN> acc[U][VEC];
+	for(int u = 0; u < U; ++u) {
+		o[u] = (og * U + u) * P.OS + os;
+		int64_t const oc = o[u] < P.O ? o[u] : P.O - 1;
+		pa[u] = A + oc * P.sAo + (kb + ks) * P.sAk + ic;
+	}
+	Sum<Acc, KAHAN> acc[U][VEC];
 
-		if(active) {
-			int64_t k = kb + ks;
-			for(; k + (KU - 1) * P.KS < ke; k += KU * P.KS) {
-				Acc bk[KU];
-				Vec<S, VEC> raw[KU][U];
+	if(active) {
+		int64_t k = kb + ks;
+		for(; k + (KU - 1) * P.KS < ke; k += KU * P.KS) {
+			Acc bk[KU];
+			Vec<S, VEC> raw[KU][U];
 #pragma unroll
-				for(int j = 0; j < KU; ++j) bk[j] = b[k + j * P.KS];
+			for(int j = 0; j < KU; ++j) bk[j] = b[k + j * P.KS];
 #pragma unroll
-				for(int j = 0; j < KU; ++j)
+			for(int j = 0; j < KU; ++j)
 #pragma unroll
-					for(int u = 0; u < U; ++u) raw[j][u].load(pa[u] + j * kstep);
+				for(int u = 0; u < U; ++u) raw[j][u].load(pa[u] + j * kstep);
 #pragma unroll
-				for(int j = 0; j < KU; ++j)
-#pragma unroll
-					for(int u = 0; u < U; ++u) {
-						Acc x[VEC];
-						decode(raw[j][u], x);
-#pragma unroll
-						for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk[j]);
-					}
-#pragma unroll
-				for(int u = 0; u < U; ++u) pa[u] += KU * kstep;
-			}
-			for(; k < ke; k += P.KS) {
-				Acc const bk = b[k];
+			for(int j = 0; j < KU; ++j)
 #pragma unroll
 				for(int u = 0; u < U; ++u) {
-					Vec<S, VEC> raw;
-					raw.load(pa[u]);
 					Acc x[VEC];
-					decode(raw, x);
+					decode(raw[j][u], x);
 #pragma unroll
-					for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk);
-					pa[u] += kstep;
+					for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk[j]);
 				}
+#pragma unroll
+			for(int u = 0; u < U; ++u) pa[u] += KU * kstep;
+		}
+		for(; k < ke; k += P.KS) {
+			Acc const bk = b[k];
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				Vec<S, VEC> raw;
+				raw.load(pa[u]);
+				Acc x[VEC];
+				decode(raw, x);
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk);
+				pa[u] += kstep;
 			}
 		}
+	}
 
-		Acc v[U][VEC];
+	Acc v[U][VEC];
+#pragma unroll
+	for(int u = 0; u < U; ++u)
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) v[u][e] = acc[u][e].value();
+
+	if(P.red == 1) {
+		// lanes it + IT*ks of one wave hold the same outputs
+		for(int d = P.IT; d < P.IT * P.KS; d <<= 1)
+#pragma unroll
+			for(int u = 0; u < U; ++u)
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) v[u][e] += __shfl_xor(v[u][e], d, 64);
+	} else if(P.red == 2) {
 #pragma unroll
 		for(int u = 0; u < U; ++u)
 #pragma unroll
-			for(int e = 0; e < VEC; ++e) v[u][e] = acc[u][e].value();
-
-		if(P.red == 1) {
-			// lanes it + IT*ks of one wave hold the same outputs
-			for(int d = P.IT; d < P.IT * P.KS; d <<= 1)
+			for(int e = 0; e < VEC; ++e) red[(u * VEC + e) * kBlock + t] = v[u][e];
+		__syncthreads();
+		if(active && ks == 0) {
+			for(int s = 1; s < P.KS; ++s)
 #pragma unroll
 				for(int u = 0; u < U; ++u)
 #pragma unroll
-					for(int e = 0; e < VEC; ++e) v[u][e] += __shfl_xor(v[u][e], d, 64);
-		} else if(P.red == 2) {
-#pragma unroll
-			for(int u = 0; u < U; ++u)
-#pragma unroll
-				for(int e = 0; e < VEC; ++e) red[(u * VEC + e) * kBlock + t] = v[u][e];
-			__syncthreads();
-			if(active && ks == 0) {
-				for(int s = 1; s < P.KS; ++s)
-#pragma unroll
-					for(int u = 0; u < U; ++u)
-#pragma unroll
-						for(int e = 0; e < VEC; ++e) v[u][e] += red[(u * VEC + e) * kBlock + t + s * P.IT];
-			}
-			__syncthreads();
+					for(int e = 0; e < VEC; ++e) v[u][e] += red[(u * VEC + e) * kBlock + t + s * P.IT];
 		}
+		__syncthreads();
+	}
 
-		if(active && ks == 0 && i < P.I) {
+	if(active && ks == 0 && i < P.I) {
 #pragma unroll
-			for(int u = 0; u < U; ++u)
-				if(o[u] < P.O)
+		for(int u = 0; u < U; ++u)
+			if(o[u] < P.O)
 #pragma unroll
-					for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, o[u], i + e, v[u][e]);
-		}
+				for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, o[u], i + e, v[u][e]);
 	}
 }
 
-// ---- k_dot ----------------------------------------------------------------------------------
-// Wave w handles row groups g = w, w + #waves, ...; a group is U*RPW rows (RPW = 64/LPR).
-// Lane l: row r = l / LPR inside each of the U row sets, k offset (l % LPR)*VEC.
+template<typename S, int VEC, int U, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                void* __restrict__ C, Args P)
+{
+	using Acc = typename Traits<S>::acc;
+	extern __shared__ unsigned char smem_raw[];
+	Acc* red = reinterpret_cast<Acc*>(smem_raw);
+	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
+	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
+	for(int64_t item = blockIdx.x; item < P.items; item += gridDim.x)
+		col_item<S, VEC, U, KAHAN>(A, b, C, P, item, kb, ke, red);
+}
+
+// ---- row tiles (q fastest) ----------------------------------------------------------------
+// A wave handles a row group g: U*RPW rows (RPW = 64/LPR).  Lane l: row r = l / LPR inside each
+// of the U row sets, k offset (l % LPR)*VEC.  Rows are reduced with butterflies over LPR lanes.
+constexpr int kDotBreg = 4;    // vector steps of b a lane keeps in registers for short rows
+
+template<typename S, int VEC>
+__device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __restrict__ b, const Args& P, int64_t kb,
+                                           int64_t ke, typename Traits<S>::acc (&bb)[kDotBreg][VEC])
+{
+	using Acc = typename Traits<S>::acc;
+	int const kl = static_cast<int>(threadIdx.x) & (P.LPR - 1);
+	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
+	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
+#pragma unroll
+	for(int s = 0; s < kDotBreg; ++s)
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) {
+			int64_t const k = k0 + s * kstride + e;
+			bb[s][e] = (s < P.kiter && k < ke) ? b[k] : Acc(0);
+		}
+}
+
+// short rows: every k step of the lane uses b from registers (P.kiter <= kDotBreg)
+template<typename S, int VEC, int U, bool KAHAN>
+__device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* __restrict__ C, const Args& P, int64_t g,
+                                                int64_t kb, int64_t ke,
+                                                const typename Traits<S>::acc (&bb)[kDotBreg][VEC])
+{
+	using Acc = typename Traits<S>::acc;
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	int const r    = lane >> P.lprShift;
+	int const kl   = lane & (P.LPR - 1);
+	int const rpw  = 64 >> P.lprShift;
+	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
+	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
+	int64_t row[U];
+	const S* pa[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) {
+		row[u] = (g * U + u) * rpw + r;
+		int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
+		pa[u] = A + rc * P.sAo + k0;
+	}
+	Sum<Acc, KAHAN> acc[U];
+#pragma unroll
+	for(int s = 0; s < kDotBreg; ++s) {
+		if(s < P.kiter && k0 + s * kstride < ke) {
+			Vec<S, VEC> raw[U];
+#pragma unroll
+			for(int u = 0; u < U; ++u) raw[u].load(pa[u] + s * kstride);
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				Acc x[VEC];
+				decode(raw[u], x);
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bb[s][e]);
+			}
+		}
+	}
+	Acc v[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
+	for(int d = 1; d < P.LPR; d <<= 1)
+#pragma unroll
+		for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
+	if(kl == 0) {
+#pragma unroll
+		for(int u = 0; u < U; ++u)
+			if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
+	}
+}
+
+// long rows: b is re-read through the cache, loads of U rows and two k steps overlap
+template<typename S, int VEC, int U, bool KAHAN>
+__device__ __forceinline__ void dot_group_long(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                               void* __restrict__ C, const Args& P, int64_t g, int64_t kb, int64_t ke)
+{
+	using Acc = typename Traits<S>::acc;
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	int const r    = lane >> P.lprShift;
+	int const kl   = lane & (P.LPR - 1);
+	int const rpw  = 64 >> P.lprShift;
+	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
+	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
+	int64_t row[U];
+	const S* pa[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) {
+		row[u] = (g * U + u) * rpw + r;
+		int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
+		pa[u] = A + rc * P.sAo;
+	}
+	Sum<Acc, KAHAN> acc[U];
+	int64_t k = k0;
+	for(; k + kstride < ke; k += 2 * kstride) {
+		Vec<S, VEC> raw[2][U];
+		Acc bv[2][VEC];
+#pragma unroll
+		for(int h = 0; h < 2; ++h)
+#pragma unroll
+			for(int u = 0; u < U; ++u) raw[h][u].load(pa[u] + k + h * kstride);
+#pragma unroll
+		for(int h = 0; h < 2; ++h)
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) bv[h][e] = b[k + h * kstride + e];
+#pragma unroll
+		for(int h = 0; h < 2; ++h)
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				Acc x[VEC];
+				decode(raw[h][u], x);
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[h][e]);
+			}
+	}
+	if(k < ke) {
+		Acc bv[VEC];
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) bv[e] = b[k + e];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			Vec<S, VEC> raw;
+			raw.load(pa[u] + k);
+			Acc x[VEC];
+			decode(raw, x);
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[e]);
+		}
+	}
+	Acc v[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
+	for(int d = 1; d < P.LPR; d <<= 1)
+#pragma unroll
+		for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
+	if(kl == 0) {
+#pragma unroll
+		for(int u = 0; u < U; ++u)
+			if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
+	}
+}
+
 template<typename S, int VEC, int U, bool KAHAN>
 __global__ __launch_bounds__(kBlock) void k_dot(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
                                                 void* __restrict__ C, Args P)
 {
 	using Acc = typename Traits<S>::acc;
-	constexpr int BREG = 4;                       // vector steps of b cached in registers
-	int const lane = static_cast<int>(threadIdx.x) & 63;
-	int const r    = lane >> P.lprShift;
-	int const kl   = lane & (P.LPR - 1);
-	int const rpw  = 64 >> P.lprShift;
 	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
 	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
-	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
 	int64_t const wave  = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
 	int64_t const waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
-	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
-
-	if(P.kiter <= BREG) {
-		// short rows: this lane's b entries stay in registers for the whole launch
-		Acc bb[BREG][VEC];
-#pragma unroll
-		for(int s = 0; s < BREG; ++s)
-#pragma unroll
-			for(int e = 0; e < VEC; ++e) {
-				int64_t const k = k0 + s * kstride + e;
-				bb[s][e] = (s < P.kiter && k < ke) ? b[k] : Acc(0);
-			}
-		for(int64_t g = wave; g < P.groups; g += waves) {
-			int64_t row[U];
-			const S* pa[U];
-#pragma unroll
-			for(int u = 0; u < U; ++u) {
-				row[u] = (g * U + u) * rpw + r;
-				int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
-				pa[u] = A + rc * P.sAo + k0;
-			}
-			Sum<Acc, KAHAN> acc[U];
-#pragma unroll
-			for(int s = 0; s < BREG; ++s) {
-				if(s < P.kiter) {
-					bool const live = k0 + s * kstride < ke;
-					Vec<S, VEC> raw[U];
-					if(live) {
-#pragma unroll
-						for(int u = 0; u < U; ++u) raw[u].load(pa[u] + s * kstride);
-#pragma unroll
-						for(int u = 0; u < U; ++u) {
-							Acc x[VEC];
-							decode(raw[u], x);
-#pragma unroll
-							for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bb[s][e]);
-						}
-					}
-				}
-			}
-			Acc v[U];
-#pragma unroll
-			for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-			for(int d = 1; d < P.LPR; d <<= 1)
-#pragma unroll
-				for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
-			if(kl == 0) {
-#pragma unroll
-				for(int u = 0; u < U; ++u)
-					if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
-			}
-		}
+	if(P.kiter <= kDotBreg) {
+		Acc bb[kDotBreg][VEC];
+		dot_load_b<S, VEC>(b, P, kb, ke, bb);
+		for(int64_t g = wave; g < P.groups; g += waves) dot_group_short<S, VEC, U, KAHAN>(A, C, P, g, kb, ke, bb);
 	} else {
-		// long rows: b is re-read through the cache, loads of U rows overlap
-		for(int64_t g = wave; g < P.groups; g += waves) {
-			int64_t row[U];
-			const S* pa[U];
-#pragma unroll
-			for(int u = 0; u < U; ++u) {
-				row[u] = (g * U + u) * rpw + r;
-				int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
-				pa[u] = A + rc * P.sAo;
-			}
-			Sum<Acc, KAHAN> acc[U];
-			int64_t k = k0;
-			for(; k + kstride < ke; k += 2 * kstride) {
-				Vec<S, VEC> raw[2][U];
-				Acc bv[2][VEC];
-#pragma unroll
-				for(int h = 0; h < 2; ++h)
-#pragma unroll
-					for(int u = 0; u < U; ++u) raw[h][u].load(pa[u] + k + h * kstride);
-#pragma unroll
-				for(int h = 0; h < 2; ++h)
-#pragma unroll
-					for(int e = 0; e < VEC; ++e) bv[h][e] = b[k + h * kstride + e];
-#pragma unroll
-				for(int h = 0; h < 2; ++h)
-#pragma unroll
-					for(int u = 0; u < U; ++u) {
-						Acc x[VEC];
-						decode(raw[h][u], x);
-#pragma unroll
-						for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[h][e]);
-					}
-			}
-			if(k < ke) {
-				Acc bv[VEC];
-#pragma unroll
-				for(int e = 0; e < VEC; ++e) bv[e] = b[k + e];
-#pragma unroll
-				for(int u = 0; u < U; ++u) {
-					Vec<S, VEC> raw;
-					raw.load(pa[u] + k);
-					Acc x[VEC];
-					decode(raw, x);
-#pragma unroll
-					for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[e]);
-				}
-			}
-			Acc v[U];
-#pragma unroll
-			for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-			for(int d = 1; d < P.LPR; d <<= 1)
-#pragma unroll
-				for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
-			if(kl == 0) {
-#pragma unroll
-				for(int u = 0; u < U; ++u)
-					if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
-			}
-		}
+		for(int64_t g = wave; g < P.groups; g += waves) dot_group_long<S, VEC, U, KAHAN>(A, b, C, P, g, kb, ke);
 	}
 }
 
