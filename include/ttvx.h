@@ -170,6 +170,21 @@ int ttvx_features(int dtype, size_t q, size_t p, const size_t* na, const size_t*
 size_t ttvx_selector_num_params(void);
 int ttvx_selector_set_weights(const float* params, size_t count);
 
+/* ---- selector network on the GPU (no counterpart in the reference) --------------------- */
+/* Device-pointer kernels that train and evaluate the selector MLP (parameter layout above).
+ * work: ttvx_mlp_work_floats(batch) floats holding the activations of the last forward pass. */
+size_t ttvx_mlp_work_floats(int batch);
+/* y[batch][12] = MLP(x[batch][24]). */
+int ttvx_mlp_forward(const float* params, const float* x, int batch, float* y, float* work, void* stream);
+/* Masked mean-squared error 0.5*sum(m*(y-t)^2)/batch into *loss and its gradient w.r.t. the
+ * trainable parameters (everything after mean/inv_std, in parameter order) into grad. */
+int ttvx_mlp_backward(const float* params, const float* y, const float* t, const float* m, int batch, float* work,
+                      float* grad, float* loss, void* stream);
+/* One Adam step (bias-corrected, step >= 1, decoupled weight decay) on the trainable parameters;
+ * m1/m2 are the moment buffers. */
+int ttvx_mlp_adam(float* params, const float* grad, float* m1, float* m2, int step, float lr, float beta1, float beta2,
+                  float eps, float weight_decay, void* stream);
+
 /* ---- batched products (no counterpart in the reference: it runs one product per call) ---- */
 /* One FP32 problem of a batch; C's extents and layout are derived from A as in the reference
  * (shape.h:103-158, layout.h:143-207); wc = NULL means C's dense strides in that layout. */
@@ -200,6 +215,9 @@ int ttvx_batch_destroy(void* batch);
 int ttvx_bench_read(const void* src, size_t bytes, float* partial, size_t partial_count, void* stream);
 /* Plain copy of `bytes` (16-byte aligned): a known-size kernel for counter calibration. */
 int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream);
+/* Bounded busy wait of one wave (iterations <= 100000 sleeps of ~8k cycles): keeps the queue
+ * busy while the host enqueues timed launches. */
+int ttvx_bench_spin(int iterations, int* sink, void* stream);
 
 #ifdef __cplusplus
 } /* extern "C" */
