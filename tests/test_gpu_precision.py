@@ -1,0 +1,119 @@
+"""Reduced-precision storage of A on the GPU.
+
+* The BF16 and E4M3 quantisers are bit-exact against the oracle's restatements
+  (oracle.bf16_rne, oracle.quantize_fp8), codes and scales alike.
+* The gfx950 E4M3 decoder (v_cvt_pk_f32_fp8 in the product kernel) reproduces the format's
+  definition for all 256 codes (read back through a product with n_q = 1).
+* BF16/FP8 products match the FP64 oracle on the dequantised values within FP32 accumulation
+  error (relative L2 <= 1e-5), with and without Kahan; the error against the original FP32
+  values is the storage error and is reported, not asserted beyond loose sanity bounds.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _flat(C):
+    span = 1 + sum((n - 1) * w for n, w in zip(C.shape, C.stride()))
+    return C.as_strided([span], [1]).cpu().numpy()
+
+
+def test_bf16_quantiser_bit_exact(ttvx, device):
+    import torch
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(1 << 20) * rng.choice([1e-3, 1, 1e3], 1 << 20)).astype(np.float32)
+    x[:6] = [0.0, -0.0, 1.00390625, 1.01171875, np.inf, -np.inf]
+    q = ttvx.quantize_bf16(torch.from_numpy(x).to(device))
+    got = q.cpu().numpy().view(np.uint16)
+    assert np.array_equal(got, O.bf16_rne(x))
+
+
+@pytest.mark.parametrize("na,pia,q,fps", [([64, 33, 20], [1, 2, 3], 2, 8), ([64, 33, 20], [3, 2, 1], 1, 1),
+                                          ([128, 16, 16], [2, 3, 1], 3, 64), ([4096, 4, 8], [1, 2, 3], 1, 1)])
+def test_fp8_quantiser_bit_exact(ttvx, device, na, pia, q, fps):
+    import torch
+    rng = np.random.default_rng(1)
+    wa = O.strides(na, pia)
+    a = rng.standard_normal(O.span(na, wa)).astype(np.float32)
+    a[rng.uniform(size=a.size) < 0.01] *= 100.0
+    A = torch.from_numpy(a).to(device).as_strided(na, wa)
+    codes, scales, f = ttvx.quantize_fp8(q, A, fps)
+    assert f == fps
+    oc, osc = O.quantize_fp8(q, a, na, wa, pia, fps)
+    assert np.array_equal(scales.cpu().numpy(), osc)
+    assert np.array_equal(codes.as_strided([a.size], [1]).cpu().numpy(), oc)
+
+
+def test_e4m3_hardware_decode_all_codes(ttvx, device):
+    """A = all 256 codes as a 256 x 1 tensor, q = 2 (n_q = 1), b = 1, scale = 1: C = decode(code)."""
+    import torch
+    codes = torch.arange(256, dtype=torch.int32).to(torch.uint8).to(device).as_strided([256, 1], [1, 256])
+    scales = torch.ones(256, dtype=torch.float32, device=device)
+    b = torch.ones(1, dtype=torch.float32, device=device)
+    C = ttvx.ttv_fp8(2, codes, scales, 1, b)
+    got = C.cpu().numpy()
+    ref = O.e4m3_decode(np.arange(256, dtype=np.uint8))
+    fin = np.isfinite(ref)
+    assert np.array_equal(got[fin], ref[fin])
+    assert np.all(np.isnan(got[~fin]))
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "fp8"])
+def test_quantised_products_vs_oracle(ttvx, device, fmt):
+    import torch
+    rng = np.random.default_rng(2)
+    for na in ([64, 48, 40], [32, 16, 24, 8], [4096, 4, 8, 4]):
+        p = len(na)
+        for pia in (list(range(1, p + 1)), list(range(p, 0, -1)), [int(x) for x in rng.permutation(p) + 1]):
+            wa = O.strides(na, pia)
+            a = rng.standard_normal(O.span(na, wa)).astype(np.float32)
+            a[rng.uniform(size=a.size) < 0.01] *= 100.0
+            Ad = torch.from_numpy(a).to(device).as_strided(na, wa)
+            for q in range(1, p + 1):
+                b = rng.uniform(-1, 1, na[q - 1]).astype(np.float32)
+                bd = torch.from_numpy(b).to(device)
+                ref = O.naive(q, a, na, wa, b, pia=pia)
+                if fmt == "bf16":
+                    Aq = ttvx.quantize_bf16(Ad)
+                    deq = O.bf16_to_f32(O.bf16_rne(a))
+                    ref_q = O.naive(q, deq, na, wa, b, pia=pia)
+                    runs = [ttvx.ttv_bf16(q, Aq, bd, flags=f) for f in (0, ttvx.FLAG_KAHAN)]
+                    storage_bound = 1e-2
+                else:
+                    codes, scales, fps = ttvx.quantize_fp8(q, Ad)
+                    table = O.e4m3_decode(np.arange(256, dtype=np.uint8))
+                    deq = table[codes.as_strided([a.size], [1]).cpu().numpy()]
+                    ref_q = O.naive(q, deq, na, wa, b, pia=pia)
+                    ref_q *= scales.cpu().numpy().astype(np.float64)[np.arange(ref_q.size) // fps]
+                    runs = [ttvx.ttv_fp8(q, codes, scales, fps, bd, flags=f) for f in (0, ttvx.FLAG_KAHAN)]
+                    storage_bound = 6e-2
+                for C in runs:
+                    got = _flat(C)
+                    assert _rel(got, ref_q) <= 1e-5, (fmt, na, pia, q)
+                    assert _rel(got, ref) <= storage_bound, (fmt, na, pia, q)
+
+
+def test_kahan_long_contraction(ttvx, device):
+    """n_q = 2^22 with a large common offset: compensated summation stays at the FP64 answer
+    while plain FP32 accumulation drifts."""
+    import torch
+    n = 1 << 22
+    rng = np.random.default_rng(3)
+    a = (1.0 + rng.uniform(-1e-3, 1e-3, n)).astype(np.float32)
+    b = np.ones(n, np.float32)
+    A = torch.from_numpy(a).to(device).as_strided([1, n], [n, 1])
+    ref = O.naive(2, a, [1, n], [n, 1], b, pia=[1, 2])
+    bd = torch.from_numpy(b).to(device)
+    plain = _flat(ttvx.ttv(2, A, bd))
+    kahan = _flat(ttvx.ttv(2, A, bd, flags=ttvx.FLAG_KAHAN))
+    e_plain, e_kahan = _rel(plain, ref), _rel(kahan, ref)
+    assert e_kahan <= 1e-7, (e_plain, e_kahan)
+    assert e_kahan <= e_plain

@@ -48,7 +48,23 @@ _SIGS = {
     "ttvx_selector_set_weights": ([ctypes.POINTER(ctypes.c_float), _sz], _int),
     "ttvx_bench_read": ([_vp, _sz, _vp, _sz, _vp], _int),
     "ttvx_bench_copy": ([_vp, _vp, _sz, _vp], _int),
+    "ttvx_bench_spin": ([_int, _vp, _vp], _int),
+    "ttvx_mlp_work_floats": ([_int], _sz),
+    "ttvx_mlp_forward": ([_vp, _vp, _int, _vp, _vp, _vp], _int),
+    "ttvx_mlp_backward": ([_vp, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp], _int),
+    "ttvx_mlp_adam": ([_vp, _vp, _vp, _vp, _int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                       ctypes.c_float, _vp], _int),
+    "ttvx_batch_create": ([_sz, _vp, ctypes.POINTER(ctypes.c_void_p)], _int),
+    "ttvx_batch_launch": ([_vp, _int, _vp], _int),
+    "ttvx_batch_info": ([_vp, ctypes.POINTER(ctypes.c_int64)], _int),
+    "ttvx_batch_destroy": ([_vp], _int),
 }
+
+
+class BatchProblem(ctypes.Structure):
+    """ttvx_batch_problem (ttvx.h)."""
+    _fields_ = [("q", ctypes.c_size_t), ("p", ctypes.c_size_t), ("a", ctypes.c_void_p), ("na", _szp),
+                ("wa", _szp), ("pia", _szp), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p), ("wc", _szp)]
 
 _lib = None
 

@@ -44,8 +44,8 @@ __device__ __forceinline__ void batch_dot_tile(const BatchDesc& d, int64_t item)
 {
 	int64_t const g = item * (kBlock / 64) + (threadIdx.x >> 6);
 	if(g >= d.args.groups) return;
-	if(d.args.kiter <= kDotBreg) {
-		float bb[kDotBreg][VEC];
+	if(d.args.kiter <= dot_breg(VEC)) {
+		float bb[dot_breg(VEC)][VEC];
 		dot_load_b<float, VEC>(d.b, d.args, 0, d.args.K, bb);
 		dot_group_short<float, VEC, kBatchDotU, false>(d.a, d.c, d.args, g, 0, d.args.K, bb);
 	} else {

@@ -198,7 +198,10 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
                                          typename Traits<S>::acc* red)
 {
 	using Acc = typename Traits<S>::acc;
-	constexpr int KU = U >= 4 ? 2 : (U == 2 ? 4 : 8);   // k unroll: U*KU independent loads in flight
+	// k unroll: U*KU independent 16-byte loads in flight per lane (8 for 4- and 8-byte elements,
+	// 4 for the narrow types, whose VEC accumulators already take the registers)
+	constexpr int KL = sizeof(S) >= 4 ? 8 : 4;
+	constexpr int KU = KL / U > 1 ? KL / U : 1;
 	int const t  = static_cast<int>(threadIdx.x);
 	int const it = t % P.IT;
 	int const ks = (t / P.IT) % P.KS;
@@ -272,19 +275,30 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 #pragma unroll
 				for(int e = 0; e < VEC; ++e) v[u][e] += __shfl_xor(v[u][e], d, 64);
 	} else if(P.red == 2) {
+		// partial sums meet in LDS; every thread then finishes some of the item's outputs
+		// (output r = ((os*U + u)*IT + it)*VEC + e, partials added in ascending ks)
 #pragma unroll
 		for(int u = 0; u < U; ++u)
 #pragma unroll
 			for(int e = 0; e < VEC; ++e) red[(u * VEC + e) * kBlock + t] = v[u][e];
 		__syncthreads();
-		if(active && ks == 0) {
-			for(int s = 1; s < P.KS; ++s)
-#pragma unroll
-				for(int u = 0; u < U; ++u)
-#pragma unroll
-					for(int e = 0; e < VEC; ++e) v[u][e] += red[(u * VEC + e) * kBlock + t + s * P.IT];
+		int const per = P.OS * U * P.IT * VEC;
+		for(int r = t; r < per; r += kBlock) {
+			int const e  = r % VEC;
+			int const r1 = r / VEC;
+			int const ir = r1 % P.IT;
+			int const r2 = r1 / P.IT;
+			int const ur = r2 % U;
+			int const orr = r2 / U;
+			Acc s = Acc(0);
+			const Acc* src = red + (ur * VEC + e) * kBlock + orr * P.IT * P.KS + ir;
+			for(int q = 0; q < P.KS; ++q) s += src[q * P.IT];
+			int64_t const oo = (og * U + ur) * P.OS + orr;
+			int64_t const ii = (itile * P.IT + ir) * VEC + e;
+			if(oo < P.O && ii < P.I) store_out<Acc>(C, P, oo, ii, s);
 		}
 		__syncthreads();
+		return;
 	}
 
 	if(active && ks == 0 && i < P.I) {
@@ -312,18 +326,19 @@ __global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const t
 // ---- row tiles (q fastest) ----------------------------------------------------------------
 // A wave handles a row group g: U*RPW rows (RPW = 64/LPR).  Lane l: row r = l / LPR inside each
 // of the U row sets, k offset (l % LPR)*VEC.  Rows are reduced with butterflies over LPR lanes.
-constexpr int kDotBreg = 4;    // vector steps of b a lane keeps in registers for short rows
+// vector steps of b a lane keeps in registers for short rows: 16 values of b whatever the type
+__host__ __device__ constexpr int dot_breg(int vec) { return vec >= 16 ? 1 : (vec >= 8 ? 2 : 4); }
 
 template<typename S, int VEC>
 __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __restrict__ b, const Args& P, int64_t kb,
-                                           int64_t ke, typename Traits<S>::acc (&bb)[kDotBreg][VEC])
+                                           int64_t ke, typename Traits<S>::acc (&bb)[dot_breg(VEC)][VEC])
 {
 	using Acc = typename Traits<S>::acc;
 	int const kl = static_cast<int>(threadIdx.x) & (P.LPR - 1);
 	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
 	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
 #pragma unroll
-	for(int s = 0; s < kDotBreg; ++s)
+	for(int s = 0; s < dot_breg(VEC); ++s)
 #pragma unroll
 		for(int e = 0; e < VEC; ++e) {
 			int64_t const k = k0 + s * kstride + e;
@@ -331,11 +346,11 @@ __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __rest
 		}
 }
 
-// short rows: every k step of the lane uses b from registers (P.kiter <= kDotBreg)
+// short rows: every k step of the lane uses b from registers (P.kiter <= dot_breg(VEC))
 template<typename S, int VEC, int U, bool KAHAN>
 __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* __restrict__ C, const Args& P, int64_t g,
                                                 int64_t kb, int64_t ke,
-                                                const typename Traits<S>::acc (&bb)[kDotBreg][VEC])
+                                                const typename Traits<S>::acc (&bb)[dot_breg(VEC)][VEC])
 {
 	using Acc = typename Traits<S>::acc;
 	int const lane = static_cast<int>(threadIdx.x) & 63;
@@ -354,7 +369,7 @@ __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* _
 	}
 	Sum<Acc, KAHAN> acc[U];
 #pragma unroll
-	for(int s = 0; s < kDotBreg; ++s) {
+	for(int s = 0; s < dot_breg(VEC); ++s) {
 		if(s < P.kiter && k0 + s * kstride < ke) {
 			Vec<S, VEC> raw[U];
 #pragma unroll
@@ -460,8 +475,8 @@ __global__ __launch_bounds__(kBlock) void k_dot(const S* __restrict__ A, const t
 	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
 	int64_t const wave  = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
 	int64_t const waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
-	if(P.kiter <= kDotBreg) {
-		Acc bb[kDotBreg][VEC];
+	if(P.kiter <= dot_breg(VEC)) {
+		Acc bb[dot_breg(VEC)][VEC];
 		dot_load_b<S, VEC>(b, P, kb, ke, bb);
 		for(int64_t g = wave; g < P.groups; g += waves) dot_group_short<S, VEC, U, KAHAN>(A, C, P, g, kb, ke, bb);
 	} else {

@@ -98,7 +98,7 @@ namespace
 const char* const kVariantNames[V_COUNT] = {
 	"dot_u1", "dot_u2", "dot_u4", "dot_u8", "dot_split",
 	"col_u1", "col_u2", "col_u4", "col_narrow", "col_split",
-	"col_k_lanes", "strided",
+	"col_k_lanes", "strided", "col_wide",
 };
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -203,17 +203,22 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		P.kiter  = static_cast<int>(std::min<int64_t>(1 << 30, ceil_div(std::min(kchunk, f.K), kstride)));
 		cfg.ksplit = static_cast<int>(ksplit);
 		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, kBlock / 64), gridcap)));
-	} else if(v >= V_COL_U1 && v <= V_COL_SPLIT) {
+	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE) {
 		if(!col_shape) return false;
 		cfg.kind = KIND_COL;
 		cfg.vec  = pick_vec(dt, a_addr, f.I, {f.K > 1 ? f.sAk : 0, f.O > 1 ? f.sAo : 0});
 		int64_t const iv = f.I / cfg.vec;
 		int64_t const itmax = v == V_COL_NARROW ? 64 : kBlock;
 		P.IT = static_cast<int>(std::min<int64_t>(iv, itmax));
-		P.KS = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kBlock / P.IT, f.K)));
+		// k groups per block: "wide" spreads the whole block over k (a flat walk through each
+		// slab); the other column variants keep >= 16 k steps per lane and fill the block with slabs
+		int64_t const ks_cap = kBlock / P.IT;
+		int64_t ks_deep = std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K / 16));
+		if(is_pow2(P.IT)) ks_deep = int64_t{1} << ilog2(ks_deep);     // keeps IT*KS <= 64 on the butterfly
+		P.KS = static_cast<int>(v == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
 		static const int us[] = {1, 2, 4, 1, 1};
-		cfg.U = us[v - V_COL_U1];
+		cfg.U = v == V_COL_WIDE ? 1 : us[v - V_COL_U1];
 		P.iTiles = ceil_div(iv, P.IT);
 		P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS) * cfg.U) * P.iTiles;
 		int64_t ksplit = 1;

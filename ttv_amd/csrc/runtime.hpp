@@ -41,7 +41,7 @@ enum Kind { KIND_DOT = 0, KIND_COL = 1, KIND_STRIDED = 2 };
 enum Variant {
 	V_DOT_U1 = 0, V_DOT_U2, V_DOT_U4, V_DOT_U8, V_DOT_SPLIT,
 	V_COL_U1, V_COL_U2, V_COL_U4, V_COL_NARROW, V_COL_SPLIT,
-	V_COL_K_LANES, V_STRIDED,
+	V_COL_K_LANES, V_STRIDED, V_COL_WIDE,
 	V_COUNT
 };
 const char* variant_name(int v);
