@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile", action="store_true",
                     help="short fixed dispatch sequence for rocprofv3: copy and read streams, then each of the "
@@ -100,7 +101,7 @@ def main():
     if args.profile:
         L = ttv_amd.lib()
         dst = torch.empty_like(Aflat)
-        partial = torch.empty(256 * 8, dtype=torch.float32, device=dev)
+        partial = torch.empty(256 * 32, dtype=torch.float32, device=dev)
         for _ in range(3):
             ttv_amd.check(L.ttvx_bench_copy(Aflat.data_ptr(), dst.data_ptr(), N * 4, stream.cuda_stream))
         for _ in range(3):
@@ -156,7 +157,7 @@ def main():
                     algorithmic_bytes_per_launch=dom["bytes"], launch_ms=round(dom["ms"], 4))
 
     # practical ceiling: pure read stream over the same buffer
-    partial = torch.empty(256 * 8, dtype=torch.float32, device=dev)
+    partial = torch.empty(256 * 32, dtype=torch.float32, device=dev)
     L = ttv_amd.lib()
     for _ in range(3):
         ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
@@ -200,13 +201,77 @@ def main():
     need_host = not (args.no_precision and args.no_cpu)
     a_host = Aflat.cpu().numpy() if need_host else None
 
+
     if not args.no_precision:
         out["precision"] = precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(np, a_host, cases, args, log)
 
+    if not args.no_batch:
+        del a_host
+        out["batching"] = batching_leg(torch, np, ttv_amd, log)
+
     print(json.dumps(out), flush=True)
+
+
+def batching_leg(torch, np, ttv_amd, log, count=512, seed=4):
+    """512 independent products (order 3/4, extents U-int[16,512], <= 2^26 elements each, random q
+    and layout) in one persistent hipGraph-replayed launch vs 512 sequential single launches."""
+    from ttv_amd.batch import Batch
+    from ttv_amd.workloads import batch_problems
+    specs = batch_problems(seed=seed, count=count)
+    dev = torch.device("cuda:0")
+    spans = [(1 + sum((n - 1) * w for n, w in zip(s["na"], s["wa"])) + 63) // 64 * 64 for s in specs]
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    pool = torch.empty(sum(spans), dtype=torch.float32, device=dev).uniform_(-1, 1, generator=g)
+    probs, off = [], 0
+    for s, sp in zip(specs, spans):
+        A = pool[off:off + sp].as_strided(s["na"], s["wa"])
+        b = torch.empty(s["na"][s["q"] - 1], dtype=torch.float32, device=dev).uniform_(-1, 1, generator=g)
+        probs.append((s["q"], A, b))
+        off += sp
+    bt = Batch(probs)
+    info = bt.info()
+    seq_out = [ttv_amd.ttv(q, A, b) for q, A, b in probs]
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    ms_graph = timed(lambda: bt.launch(graph=True), 5)
+    ms_plain = timed(lambda: bt.launch(graph=False), 5)
+
+    def sequential():
+        for (q, A, b), C in zip(probs, seq_out):
+            ttv_amd.ttv(q, A, b, out=C)
+
+    ms_seq = timed(sequential, 3)
+    worst = 0.0
+    for Cb, Cs in zip(bt.outputs, seq_out):
+        d = float(torch.linalg.vector_norm((Cb - Cs).double()) / torch.linalg.vector_norm(Cs.double()))
+        worst = max(worst, d)
+    gb = info["bytes"] / 1e9
+    res = dict(problems=count, tiles=info["tiles"], algorithmic_gb=round(gb, 2),
+               batched_graph_ms=round(ms_graph, 3), batched_ms=round(ms_plain, 3), sequential_ms=round(ms_seq, 3),
+               batched_gbs=round(gb / (ms_graph * 1e-3), 1), sequential_gbs=round(gb / (ms_seq * 1e-3), 1),
+               speedup=round(ms_seq / ms_graph, 3), max_rel_diff_vs_sequential=worst)
+    log("batching: %d problems %.1f GB  graph %.3f ms (%.0f GB/s)  plain %.3f ms  sequential %.3f ms (%.0f GB/s)  "
+        "max rel diff %.1e" % (count, gb, ms_graph, gb / (ms_graph * 1e-3), ms_plain, ms_seq, gb / (ms_seq * 1e-3), worst))
+    bt.close()
+    return res
 
 
 def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):

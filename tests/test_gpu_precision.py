@@ -101,6 +101,48 @@ def test_quantised_products_vs_oracle(ttvx, device, fmt):
                     assert _rel(got, ref) <= storage_bound, (fmt, na, pia, q)
 
 
+def test_order8_precision_config(ttvx, device):
+    """Order-8 FP32 tensor (4096,4,8,4,8,4,2,64) = 2^31 elements (8 GiB), layout drawn from a
+    seed, q in {1,4,8}; A ~ N(0,1) with 1% of entries x100; BF16 and FP8-e4m3 storage with FP32
+    accumulation (and Kahan).  Relative L2 and max errors against the FP64 oracle on the original
+    values are printed; BF16 must stay within 1e-2, FP8 within its storage error (<= 6e-2)."""
+    import torch
+    na = [4096, 4, 8, 4, 8, 4, 2, 64]
+    rng = np.random.default_rng(31)
+    pia = [int(x) for x in rng.permutation(8) + 1]
+    wa = O.strides(na, pia)
+    n = int(np.prod(na))
+    g = torch.Generator(device=device)
+    g.manual_seed(31)
+    A = torch.randn(n, generator=g, device=device)
+    A.mul_(torch.where(torch.rand(n, generator=g, device=device) < 0.01, 100.0, 1.0))
+    a = A.cpu().numpy()
+    Av = A.as_strided(na, wa)
+    Abf = ttvx.quantize_bf16(A).as_strided(na, wa)
+    for q in (1, 4, 8):
+        b = rng.uniform(-1, 1, na[q - 1]).astype(np.float32)
+        bd = torch.from_numpy(b).to(device)
+        print("order-8 config: pia", pia, "q", q, flush=True)
+        ref = O.naive(q, a, na, wa, b, pia=pia)
+        rn, rmax = np.linalg.norm(ref), np.max(np.abs(ref))
+        res = {}
+        res["fp32"] = _flat(ttvx.ttv(q, Av, bd))
+        res["bf16"] = _flat(ttvx.ttv_bf16(q, Abf, bd))
+        res["bf16_kahan"] = _flat(ttvx.ttv_bf16(q, Abf, bd, flags=ttvx.FLAG_KAHAN))
+        codes, scales, fps = ttvx.quantize_fp8(q, Av)
+        res["fp8"] = _flat(ttvx.ttv_fp8(q, codes, scales, fps, bd))
+        res["fp8_kahan"] = _flat(ttvx.ttv_fp8(q, codes, scales, fps, bd, flags=ttvx.FLAG_KAHAN))
+        del codes, scales
+        for k, got in res.items():
+            d = got.astype(np.float64) - ref
+            print("  q=%d %-10s rel L2 %.3e  max abs %.3e (max |C| %.3e)" % (q, k, np.linalg.norm(d) / rn,
+                                                                            np.max(np.abs(d)), rmax), flush=True)
+        assert np.linalg.norm(res["fp32"] - ref) <= 1e-5 * rn
+        assert np.linalg.norm(res["bf16"] - ref) <= 1e-2 * rn
+        assert np.linalg.norm(res["fp8"] - ref) <= 6e-2 * rn
+        del res, ref
+
+
 def test_kahan_long_contraction(ttvx, device):
     """n_q = 2^22 with a large common offset: compensated summation stays at the FP64 answer
     while plain FP32 accumulation drifts."""

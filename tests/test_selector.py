@@ -1,0 +1,73 @@
+"""What the learned selector gains over the fixed rule, on the CPU.
+
+ttv_amd/data/ holds the timing tables the exhaustive autotuner measured on an MI355X (every valid
+variant of 20,000 training and 2,000 held-out shapes, HBM-cold, median of 3) and the network
+trained on the GPU from the training table.  Here the library's own host-side selection
+(ttvx_plan_info under TTVX_SELECT_LEARNED, no GPU needed) picks a variant for every held-out
+shape, and its regret t(pick)/t(fastest) - 1 is compared with the fixed rule's.
+The network itself is checked against the FP64 restatement in oracle/ (oracle.mlp_forward).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DATA = os.path.join(REPO, "ttv_amd", "data")
+TEST_TABLE = os.path.join(DATA, "selector_timings_test_gfx950.npz")
+WEIGHTS = os.path.join(DATA, "selector_gfx950.npz")
+
+pytestmark = pytest.mark.skipif(not (os.path.exists(TEST_TABLE) and os.path.exists(WEIGHTS)),
+                                reason="selector weights / timing tables not generated yet")
+
+
+def _regret(choice, times):
+    best = np.nanmin(times, axis=1)
+    return float(np.mean(times[np.arange(len(choice)), choice] / best - 1.0))
+
+
+def test_learned_selector_beats_fixed_rule_on_held_out_shapes(ttvx):
+    from ttv_amd import selector as S
+    z = np.load(TEST_TABLE)
+    times = z["times"]
+    shapes = S.shapes_from_arrays(z)
+    params = S.load_weights(WEIGHTS)
+    ttvx.set_selector_weights(params)
+    ttvx.set_selection(ttvx.SELECT_LEARNED)
+    try:
+        learned = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"]
+                            for s in shapes])
+    finally:
+        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
+        ttvx.set_selector_weights(None)
+    fixed = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"]
+                      for s in shapes])
+    assert np.all(np.isfinite(times[np.arange(len(learned)), learned])), "learned pick must be a timed variant"
+    r_learned, r_fixed = _regret(learned, times), _regret(fixed, times)
+    print("held-out regret: learned %.4f, fixed rule %.4f" % (r_learned, r_fixed))
+    assert r_learned <= 0.05
+    assert r_learned < r_fixed
+
+
+def test_host_inference_matches_fp64_restatement(ttvx):
+    from ttv_amd import selector as S
+    z = np.load(TEST_TABLE)
+    params = S.load_weights(WEIGHTS)
+    feats = z["feats"][:300]
+    y64, _ = O.mlp_forward(params, feats)
+    valid = np.isfinite(z["times"][:300])
+    pick64 = np.argmin(np.where(valid, y64, np.inf), axis=1)
+    shapes = S.shapes_from_arrays(z)[:300]
+    ttvx.set_selector_weights(params)
+    ttvx.set_selection(ttvx.SELECT_LEARNED)
+    try:
+        host = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"]
+                         for s in shapes])
+        feats_now = np.stack([ttvx.features(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]]) for s in shapes])
+    finally:
+        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
+        ttvx.set_selector_weights(None)
+    assert np.array_equal(feats_now, feats)
+    assert np.mean(host == pick64) >= 0.99

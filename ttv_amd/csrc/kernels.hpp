@@ -41,32 +41,34 @@ template<> struct Traits<uint8_t>  { using acc = float;  static constexpr int vm
 // Loads VEC consecutive elements (VEC*sizeof(S) bytes, naturally aligned) and converts them to
 // the accumulation type.
 
+// A is streamed exactly once, so its loads are non-temporal (no reuse to keep in the caches);
+// measured on MI355X this lifts a pure 1 GiB read stream from ~6.0 to ~7.0 TB/s.
 template<int BYTES> struct RawLoad;
 template<> struct RawLoad<16> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[4]) {
-		u32x4 v = *reinterpret_cast<const u32x4*>(p);
+		u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
 		w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
 	}
 };
 template<> struct RawLoad<8> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[2]) {
-		u32x2 v = *reinterpret_cast<const u32x2*>(p);
+		u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
 		w[0] = v.x; w[1] = v.y;
 	}
 };
 template<> struct RawLoad<4> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = *reinterpret_cast<const uint32_t*>(p);
+		w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
 	}
 };
 template<> struct RawLoad<2> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = *reinterpret_cast<const uint16_t*>(p);
+		w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
 	}
 };
 template<> struct RawLoad<1> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = *reinterpret_cast<const uint8_t*>(p);
+		w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint8_t*>(p));
 	}
 };
 
@@ -158,8 +160,9 @@ struct Args {
 	// k_col tiling
 	int IT, KS, OS;           // lanes along i (vectors), lane groups along k, slabs per item
 	int red;                  // cross-lane reduction: 0 none, 1 butterfly in one wave, 2 LDS
-	int64_t iTiles;           // tiles along i
-	int64_t items;            // ceil(O / (OS*U)) * iTiles
+	int along_i;              // the U units of a thread lie along i (wide rows) instead of along o
+	int64_t iTiles;           // tiles along i (IT*VEC columns, or U*IT*VEC when along_i)
+	int64_t items;            // ceil(O / (OS*U)) * iTiles, or ceil(O / OS) * iTiles when along_i
 	// k_dot tiling
 	int LPR;                  // lanes per row (power of two <= 64)
 	int lprShift;             // log2(LPR)
@@ -189,9 +192,22 @@ __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, i
 }
 
 // ---- column tiles -------------------------------------------------------------------------
-// Block tile: OS*U output slabs x IT*VEC columns, all of the block's k range.
+// Block tile: OS*U output slabs x IT*VEC columns (or OS slabs x U*IT*VEC columns when the U
+// units lie along i), all of the block's k range.
 // Thread t: it = t % IT, ks = (t / IT) % KS, os = t / (IT*KS).  Every thread of the block calls
 // col_item for the same item (the LDS reduction synchronises the block).
+// Unit u of a thread: slab o_u and first column i_u of its VEC-vector.
+__device__ __forceinline__ void col_unit(const Args& P, int64_t og, int64_t itile, int os, int it, int u, int U, int VEC,
+                                         int64_t& o, int64_t& i)
+{
+	if(P.along_i) {
+		o = og * P.OS + os;
+		i = ((itile * U + u) * P.IT + it) * VEC;
+	} else {
+		o = (og * U + u) * P.OS + os;
+		i = (itile * P.IT + it) * VEC;
+	}
+}
 template<typename S, int VEC, int U, bool KAHAN>
 __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
                                          void* __restrict__ C, const Args& P, int64_t item, int64_t kb, int64_t ke,
@@ -211,14 +227,14 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 
 	int64_t const og    = item / P.iTiles;
 	int64_t const itile = item - og * P.iTiles;
-	int64_t const i     = (itile * P.IT + it) * VEC;
-	int64_t const ic    = i < P.I ? i : P.I - VEC;        // clamped: tail lanes re-read live data
-	int64_t o[U];
+	int64_t o[U], iu[U];
 	const S* pa[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) {
-		o[u] = (og * U + u) * P.OS + os;
+		col_unit(P, og, itile, os, it, u, U, VEC, o[u], iu[u]);
+		// clamped: tail lanes re-read live data and store nothing
 		int64_t const oc = o[u] < P.O ? o[u] : P.O - 1;
+		int64_t const ic = iu[u] < P.I ? iu[u] : P.I - VEC;
 		pa[u] = A + oc * P.sAo + (kb + ks) * P.sAk + ic;
 	}
 	Sum<Acc, KAHAN> acc[U][VEC];
@@ -293,20 +309,21 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 			Acc s = Acc(0);
 			const Acc* src = red + (ur * VEC + e) * kBlock + orr * P.IT * P.KS + ir;
 			for(int q = 0; q < P.KS; ++q) s += src[q * P.IT];
-			int64_t const oo = (og * U + ur) * P.OS + orr;
-			int64_t const ii = (itile * P.IT + ir) * VEC + e;
+			int64_t oo, ii;
+			col_unit(P, og, itile, orr, ir, ur, U, VEC, oo, ii);
+			ii += e;
 			if(oo < P.O && ii < P.I) store_out<Acc>(C, P, oo, ii, s);
 		}
 		__syncthreads();
 		return;
 	}
 
-	if(active && ks == 0 && i < P.I) {
+	if(active && ks == 0) {
 #pragma unroll
 		for(int u = 0; u < U; ++u)
-			if(o[u] < P.O)
+			if(o[u] < P.O && iu[u] < P.I)
 #pragma unroll
-				for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, o[u], i + e, v[u][e]);
+				for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, o[u], iu[u] + e, v[u][e]);
 	}
 }
 

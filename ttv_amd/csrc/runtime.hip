@@ -219,13 +219,22 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
 		static const int us[] = {1, 2, 4, 1, 1};
 		cfg.U = v == V_COL_WIDE ? 1 : us[v - V_COL_U1];
-		P.iTiles = ceil_div(iv, P.IT);
-		P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS) * cfg.U) * P.iTiles;
+		// wide rows (a full-width block still leaves U vectors per lane): the U units of a lane
+		// step along i, so a block streams U*IT*16 bytes of every row it touches
+		P.along_i = (cfg.U > 1 && iv >= static_cast<int64_t>(P.IT) * cfg.U && P.IT == kBlock) ? 1 : 0;
+		if(P.along_i) {
+			P.iTiles = ceil_div(iv, static_cast<int64_t>(P.IT) * cfg.U);
+			P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS)) * P.iTiles;
+		} else {
+			P.iTiles = ceil_div(iv, P.IT);
+			P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS) * cfg.U) * P.iTiles;
+		}
 		int64_t ksplit = 1;
 		if(v == V_COL_SPLIT) {
-			int64_t const want = static_cast<int64_t>(std::max(cus, 1)) * 8;
-			if(P.items >= want || f.K / P.KS < 16) return false;
-			ksplit = std::min<int64_t>(64, std::min<int64_t>(ceil_div(want, P.items), f.K / P.KS / 8));
+			// only when the output tiles cannot fill the GPU and every split keeps long k runs
+			int64_t const want = static_cast<int64_t>(std::max(cus, 1)) * 4;
+			if(P.items >= std::max(cus, 1) || f.K / P.KS < 64) return false;
+			ksplit = std::min<int64_t>(64, std::min<int64_t>(ceil_div(want, P.items), f.K / P.KS / 32));
 			if(ksplit < 2) return false;
 		}
 		int64_t const kchunk = ceil_div(f.K, ksplit);
@@ -280,9 +289,10 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 	if(f.I > 1 && f.sAi == 1 && f.sCi == 1) {
 		if(configure(V_COL_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_COL_SPLIT;
 		configure(V_COL_U1, f, dt, a_addr, cus, flags, cfg);
+		int64_t const iv = f.I / cfg.vec;
+		if(iv >= 4 * kBlock) return V_COL_U4;            // wide rows: 4 vectors per lane along i
 		int64_t const per_thread = f.K / std::max(1, cfg.args.KS);
-		if(per_thread >= 8) return V_COL_U1;
-		if(per_thread >= 4) return V_COL_U2;
+		if(per_thread >= 8) return V_COL_U2;
 		return V_COL_U4;
 	}
 	return V_STRIDED;

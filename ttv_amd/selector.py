@@ -86,10 +86,15 @@ def _launcher(pool, shape):
     return run
 
 
+STRIDED = 11
+
+
 def valid_variants(shape):
+    """Variants that can run the shape; the strided fallback only when nothing else can."""
     import ttv_amd
     code = DT_CODE[shape["dtype"]]
-    return [v for v in range(NV) if ttv_amd.variant_valid(shape["q"], shape["na"], shape["wa"], shape["pia"], v, code)]
+    vs = [v for v in range(NV) if ttv_amd.variant_valid(shape["q"], shape["na"], shape["wa"], shape["pia"], v, code)]
+    return [v for v in vs if v != STRIDED] or vs
 
 
 def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0):
@@ -144,6 +149,31 @@ def collect_timings(shapes, reps=3, log=print, pool=None):
         if log and (n + 1) % 500 == 0:
             log("timed %d/%d shapes in %.0f s" % (n + 1, len(shapes), time.time() - t0))
     return feats, times, heur
+
+
+def shape_arrays(shapes, pmax=10):
+    """Shapes as fixed-width arrays (for storing next to their timings)."""
+    n = len(shapes)
+    na = np.zeros((n, pmax), np.int64)
+    pia = np.zeros((n, pmax), np.int64)
+    meta = np.zeros((n, 3), np.int64)          # p, q, dtype code
+    for k, s in enumerate(shapes):
+        p = len(s["na"])
+        na[k, :p] = s["na"]
+        pia[k, :p] = s["pia"]
+        meta[k] = (p, s["q"], DT_CODE[s["dtype"]])
+    return dict(na=na, pia=pia, meta=meta)
+
+
+def shapes_from_arrays(z):
+    out = []
+    for k in range(z["meta"].shape[0]):
+        p, q, code = (int(x) for x in z["meta"][k])
+        na = [int(x) for x in z["na"][k, :p]]
+        pia = [int(x) for x in z["pia"][k, :p]]
+        dt = "f32" if code == _lib.F32 else "bf16"
+        out.append(dict(na=na, q=q, pia=pia, wa=generate_strides(na, pia), dtype=dt))
+    return out
 
 
 # ---- training on the GPU --------------------------------------------------------------------
@@ -282,10 +312,12 @@ def main():
     t0 = time.time()
     f_te, t_te, h_te = collect_timings(shapes_te, reps=a.reps, log=log, pool=pool)
     log("held-out timings done in %.0f s" % (time.time() - t0))
-    np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te)
+    np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te,
+                        **shape_arrays(shapes_te))
     f_tr, t_tr, h_tr = collect_timings(shapes_tr, reps=a.reps, log=log, pool=pool)
     log("training timings done in %.0f s" % (time.time() - t0))
-    np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr)
+    np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
+                        **shape_arrays(shapes_tr))
     del pool
     torch.cuda.empty_cache()
     params = train(f_tr, t_tr, epochs=a.epochs, log=log)
