@@ -22,6 +22,7 @@
 #include "detail/cases.h"
 #include "detail/index.h"
 #include "detail/layout.h"
+#include "detail/matrix_times_vector.h"
 #include "detail/shape.h"
 #include "detail/strides.h"
 #include "detail/tags.h"

@@ -127,6 +127,50 @@ static void all_policies()
 	check_policy<T>(ep::par_blas_loop, sp::slice, fp::all, "par_blas_loop/slice/all");
 }
 
+// test/src/gtest_tlib_mtv.cpp:17-180: the 2-D slice kernels on (2..2^steps)^2 shapes, row- and
+// column-major fills 1..M*N, b = 1, row sums fn(i) - fn(i-1) with fn(i) = i*n*(i*n+1)/2.
+template<class T, class F1, class F2>
+static void check_mtv(F1 col_fn, F2 row_fn, std::size_t steps, const char* name)
+{
+	for(std::size_t e0 = 0; e0 < steps; ++e0)
+		for(std::size_t e1 = 0; e1 < steps; ++e1) {
+			std::size_t const m = std::size_t{2} << e1, n = std::size_t{2} << e0;
+			std::vector<T> a(m * n), b(n, T{1}), c(m, T{0});
+			auto fn = [n](std::size_t i) { return (i * n * (i * n + 1)) / 2; };
+			auto expect_ok = [&]() {
+				bool ok = true;
+				for(std::size_t i = 1; i <= m; ++i) ok = ok && c[i - 1] == static_cast<T>(fn(i) - fn(i - 1));
+				return ok;
+			};
+			for(std::size_t j = 0; j < n; ++j)          // column-major fill
+				for(std::size_t i = 0; i < m; ++i) a[i + j * m] = static_cast<T>(i * n + j + 1);
+			col_fn(a.data(), b.data(), c.data(), m, n, m);
+			CHECK(expect_ok(), "%s column-major %zux%zu", name, m, n);
+			std::fill(c.begin(), c.end(), T{0});
+			for(std::size_t i = 0; i < m; ++i)          // row-major fill
+				for(std::size_t j = 0; j < n; ++j) a[j + i * n] = static_cast<T>(j + 1 + i * n);
+			row_fn(a.data(), b.data(), c.data(), m, n, n);
+			CHECK(expect_ok(), "%s row-major %zux%zu", name, m, n);
+		}
+}
+
+static void mtv_tests()
+{
+	using namespace tlib::ttv::detail;
+	check_mtv<double>(gemv_col<double, std::size_t>, gemv_row<double, std::size_t>, 10, "Gemv");
+	check_mtv<double>(gemv_col_parallel<double, std::size_t>, gemv_row_parallel<double, std::size_t>, 10, "GemvParallel");
+	check_mtv<float>(gemv_col_blas<float, std::size_t>, gemv_row_blas<float, std::size_t>, 8, "GemvBLAS");
+	// the non-BLAS column kernel adds into c; the BLAS one overwrites
+	std::vector<double> a = {1, 2, 3, 4}, b = {1, 1}, c = {10, 20};
+	gemv_col<double, std::size_t>(a.data(), b.data(), c.data(), 2, 2, 2);
+	CHECK(c[0] == 14 && c[1] == 26, "gemv_col accumulates");
+	gemv_col_blas<double, std::size_t>(a.data(), b.data(), c.data(), 2, 2, 2);
+	CHECK(c[0] == 4 && c[1] == 6, "gemv_col_blas overwrites");
+	double d = 0;
+	dot<double, std::size_t>(a.data(), a.data(), &d, 4);
+	CHECK(d == 30, "dot");
+}
+
 static void examples()
 {
 	// interface1.cpp: auto C1 = A(mode) * B;
@@ -197,6 +241,7 @@ static void examples()
 int main()
 {
 	examples();
+	mtv_tests();
 	all_policies<double>();
 	check_policy<float>(execution_policy::par_loop, slicing_policy::subtensor, fusion_policy::all, "float default");
 	std::printf("%ld checks, %ld failures\n", g_checks, g_fail);
