@@ -72,6 +72,15 @@ int ttvx_version(void);
 /* Number of visible HIP devices; 0 without a GPU.  Never fails. */
 int ttvx_device_count(void);
 
+/* ---- device memory for resident tensors ------------------------------------------------ */
+/* Back the device-resident tensor of include/tlib/device_tensor.h (the reference's tensor<T>,
+   include/tlib/detail/tensor.h:56-114, keeps std::vector storage; on the GPU the data stays in HBM
+   between products).  Allocation on the current device (gfx950 required); copies are synchronous
+   and take host<->device or device<->device pointers.  ttvx_device_free(NULL) is a no-op. */
+int ttvx_device_alloc(size_t bytes, void** out);
+int ttvx_device_free(void* ptr);
+int ttvx_copy(void* dst, const void* src, size_t bytes);
+
 /* ---- plain drop-in ------------------------------------------------------------------- */
 /* Replaces tlib::ttv::ttv<float,std::size_t,ep,sp,fp>(ep,sp,fp,q,p,a,na,wa,pia,b,nb,c,nc,wc,pic)
  * (reference include/tlib/ttv.h:54-92, dispatching to include/tlib/detail/tensor_times_vector.h).

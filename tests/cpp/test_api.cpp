@@ -6,6 +6,7 @@
 // library's synchronous entry points (staged to the GPU).  Also runs the three example
 // programs' computations (example/interface1.cpp, interface2.cpp, interface3.cpp) and the
 // argument checks.  Exit status 0 = all checks passed.
+#include <tlib/device_tensor.h>
 #include <tlib/ttv.h>
 
 #include <algorithm>
@@ -171,6 +172,64 @@ static void mtv_tests()
 	CHECK(d == 30, "dot");
 }
 
+// Device-resident tensors (include/tlib/device_tensor.h): the interface1 example with the data in
+// HBM, a chain of three products that never leaves the GPU (every layout of an order-4 tensor,
+// exact integer answers), moves, and the reference's error text.
+static void device_tensors()
+{
+	{
+		auto Ah = tensor<float>({4, 3, 2});
+		auto Bh = tensor<float>({3, 1});
+		std::iota(Ah.begin(), Ah.end(), 1);
+		std::fill(Bh.begin(), Bh.end(), 1);
+		device_tensor<float> A(Ah), B(Bh);
+		auto C = A(2) * B;
+		std::vector<float> expect = {15, 18, 21, 24, 51, 54, 57, 60};
+		CHECK(C.to_host().data() == expect, "device interface1");
+		CHECK((C.shape() == vec{4, 2}) && (C.layout() == vec{1, 2}), "device interface1 metadata");
+		auto D = std::move(C);
+		CHECK(C.data() == nullptr && D.to_host().data() == expect, "device tensor move");
+	}
+	vec const na = {8, 4, 6, 2};
+	vec perm = {1, 2, 3, 4};
+	do {
+		auto Ah = tensor<double>(na, perm);
+		std::iota(Ah.begin(), Ah.end(), 1.0);
+		auto const wa = Ah.strides();
+		device_tensor<double> A(Ah);
+		std::vector<device_tensor<double>> ones;
+		for(std::size_t m = 1; m < 4; ++m) {
+			auto o = tensor<double>({na[m], 1});
+			o = 1.0;
+			ones.emplace_back(o);
+		}
+		// modes 4, 3, 2 in turn: C[i] = sum over j,k,l of A(i,j,k,l)
+		auto C = ttv(2, ttv(3, ttv(4, A, ones[2]), ones[1]), ones[0]);
+		auto Ch = C.to_host();
+		bool ok = Ch.shape() == vec{8};
+		for(std::size_t i = 0; i < na[0] && ok; ++i) {
+			double want = 0;
+			for(std::size_t j = 0; j < na[1]; ++j)
+				for(std::size_t k = 0; k < na[2]; ++k)
+					for(std::size_t l = 0; l < na[3]; ++l)
+						want += 1.0 + double(i * wa[0] + j * wa[1] + k * wa[2] + l * wa[3]);
+			ok = Ch.data()[i] == want;
+		}
+		CHECK(ok, "device chain layout %zu%zu%zu%zu", perm[0], perm[1], perm[2], perm[3]);
+	} while(std::next_permutation(perm.begin(), perm.end()));
+	{
+		device_tensor<double> A(tensor<double>({4, 3, 2})), B(tensor<double>({5, 1}));
+		std::string msg;
+		try {
+			auto C = A(2) * B;
+			(void)C;
+		} catch(std::runtime_error const& e) {
+			msg = e.what();
+		}
+		CHECK(msg == "Error in tlib::tensor_times_vector: contraction dimension of A and B are not equal.", "device: %s", msg.c_str());
+	}
+}
+
 static void examples()
 {
 	// interface1.cpp: auto C1 = A(mode) * B;
@@ -242,6 +301,7 @@ int main()
 {
 	examples();
 	mtv_tests();
+	device_tensors();
 	all_policies<double>();
 	check_policy<float>(execution_policy::par_loop, slicing_policy::subtensor, fusion_policy::all, "float default");
 	std::printf("%ld checks, %ld failures\n", g_checks, g_fail);

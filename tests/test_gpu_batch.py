@@ -48,7 +48,7 @@ def test_batch_matches_single_launches_and_oracle(ttvx, device):
             na, wa = list(A.shape), list(A.stride())
             pia = ttvx.layout_from_strides(na, wa)
             info = ttvx.plan_info(q, na, wa, pia)
-            v = 5 if info["I"] > 1 else 1          # the batch's configurations: col_u1 / dot_u2
+            v = 6 if info["I"] > 1 else 2          # the batch's configurations: col_u2 / dot_u4
             ttvx.set_selection(ttvx.SELECT_FORCED, v)
             single = _flat(ttvx.ttv(q, A, b)).cpu().numpy()
             assert np.array_equal(single, batched[n]), (n, specs[n])

@@ -28,6 +28,9 @@ _SIGS = {
     "ttvx_last_error": ([], ctypes.c_char_p),
     "ttvx_version": ([], _int),
     "ttvx_device_count": ([], _int),
+    "ttvx_device_alloc": ([_sz, ctypes.POINTER(ctypes.c_void_p)], _int),
+    "ttvx_device_free": ([_vp], _int),
+    "ttvx_copy": ([_vp, _vp, _sz], _int),
     "ttvx_ttv_f32": ([_int, _int, _int, _sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp], _int),
     "ttvx_ttv_f64": ([_int, _int, _int, _sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp], _int),
     "ttvx_ttv_f32_async": ([_sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp, _uint, _vp], _int),

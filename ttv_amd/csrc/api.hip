@@ -157,6 +157,39 @@ int ttvx_device_count(void)
 	return n;
 }
 
+int ttvx_device_alloc(size_t bytes, void** out)
+{
+	clear_error();
+	if(out == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx_device_alloc: null output pointer");
+	*out = nullptr;
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	if(bytes == 0) return TTVX_OK;
+	hipError_t const e = hipMalloc(out, bytes);
+	if(e != hipSuccess) {
+		*out = nullptr;
+		return hip_fail(e, "hipMalloc");
+	}
+	return TTVX_OK;
+}
+
+int ttvx_device_free(void* ptr)
+{
+	clear_error();
+	if(ptr == nullptr) return TTVX_OK;
+	hipError_t const e = hipFree(ptr);
+	return e == hipSuccess ? TTVX_OK : hip_fail(e, "hipFree");
+}
+
+int ttvx_copy(void* dst, const void* src, size_t bytes)
+{
+	clear_error();
+	if(bytes == 0) return TTVX_OK;
+	if(dst == nullptr || src == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx_copy: null pointer");
+	hipError_t const e = hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+	return e == hipSuccess ? TTVX_OK : hip_fail(e, "hipMemcpy");
+}
+
 int ttvx_ttv_f32(int ep, int sp, int fp, size_t q, size_t p,
                  const float* a, const size_t* na, const size_t* wa, const size_t* pia,
                  const float* b, const size_t* nb,

@@ -1,15 +1,17 @@
 // Batched mode-q products: many independent TTVs of different order, shape, mode and layout in
 // one persistent launch (no counterpart in the reference, which runs one product per call).
 //
-// At creation every problem is validated and folded (plan.hpp), gets the launch configuration of
-// its fixed-rule variant restricted to the batch instantiations (column tiles with U = 1, row
-// groups with U = 2; no split of n_q), and is cut into block tiles.  Problems are ordered by tile
-// cost, largest first, and their tiles numbered consecutively.  The persistent kernel's blocks
-// take tile numbers from a device counter (dynamic load balance), find the owning problem by
-// binary search over the tile offsets and run the same tile bodies as the single-problem kernels,
-// so a problem's result does not depend on whether it ran batched or alone.  A fill kernel
-// clears the counter before every launch; both kernels can be captured once into a hipGraph
-// and replayed.
+// At creation every problem is validated and folded (plan.hpp) and gets the launch configuration
+// of the single-problem tiles the batch kernel instantiates (column tiles with U = 2, row groups
+// with U = 4; no split of n_q).  Its work units (column items, or row groups) are cut into batch
+// tiles of about kTileBytes of A each, so a tile costs one counter increment and one table
+// search however small the units are.  Problems are ordered largest first and their tiles
+// numbered consecutively.  The persistent kernel runs as many blocks as the GPU holds at once;
+// each takes tile numbers from a device counter (dynamic load balance), finds the owning problem
+// by binary search over the tile offsets and runs the same tile bodies as the single-problem
+// kernels, so a problem's result does not depend on whether it ran batched or alone.  A fill
+// kernel clears the counter before every launch; both kernels can be captured once into a
+// hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,33 +32,43 @@ struct BatchDesc {
 	const float* a;
 	const float* b;
 	float* c;
-	int64_t tile0;     // first tile number of this problem
+	int64_t units;     // column items, or row groups
+	int64_t per_tile;  // units per batch tile (a multiple of the block's waves for row groups)
 	int32_t kind;      // KIND_DOT or KIND_COL
 	int32_t vec;
 };
 
-constexpr int kBatchDotU = 2;
+constexpr int kBatchColU = 2;
+constexpr int kBatchDotU = 4;
+constexpr int64_t kTileBytes = int64_t{1} << 20;
 
 __global__ __launch_bounds__(kBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
 
 template<int VEC>
-__device__ __forceinline__ void batch_dot_tile(const BatchDesc& d, int64_t item)
+__device__ __forceinline__ void batch_dot_tile(const BatchDesc& d, int64_t gb, int64_t ge)
 {
-	int64_t const g = item * (kBlock / 64) + (threadIdx.x >> 6);
-	if(g >= d.args.groups) return;
 	if(d.args.kiter <= dot_breg(VEC)) {
 		float bb[dot_breg(VEC)][VEC];
 		dot_load_b<float, VEC>(d.b, d.args, 0, d.args.K, bb);
-		dot_group_short<float, VEC, kBatchDotU, false>(d.a, d.c, d.args, g, 0, d.args.K, bb);
+		for(int64_t g = gb + (threadIdx.x >> 6); g < ge; g += kBlock / 64)
+			dot_group_short<float, VEC, kBatchDotU, false>(d.a, d.c, d.args, g, 0, d.args.K, bb);
 	} else {
-		dot_group_long<float, VEC, kBatchDotU, false>(d.a, d.b, d.c, d.args, g, 0, d.args.K);
+		for(int64_t g = gb + (threadIdx.x >> 6); g < ge; g += kBlock / 64)
+			dot_group_long<float, VEC, kBatchDotU, false>(d.a, d.b, d.c, d.args, g, 0, d.args.K);
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* __restrict__ descs, int nprob, int64_t total_tiles,
-                                                  unsigned int* __restrict__ counter)
+template<int VEC>
+__device__ __forceinline__ void batch_col_tile(const BatchDesc& d, int64_t ib, int64_t ie, float* red)
 {
-	__shared__ float red[kBlock * 4];
+	for(int64_t item = ib; item < ie; ++item)
+		col_item<float, VEC, kBatchColU, false>(d.a, d.b, d.c, d.args, item, 0, d.args.K, red);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* __restrict__ descs, const int64_t* __restrict__ tile0,
+                                                  int nprob, int64_t total_tiles, unsigned int* __restrict__ counter)
+{
+	__shared__ float red[kBlock * 4 * kBatchColU];
 	__shared__ int64_t s_tile;
 	for(;;) {
 		if(threadIdx.x == 0) s_tile = static_cast<int64_t>(atomicAdd(counter, 1u));
@@ -64,26 +76,27 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* __restrict__ 
 		int64_t const t = s_tile;
 		__syncthreads();
 		if(t >= total_tiles) break;
-		// owning problem: last descriptor with tile0 <= t
+		// owning problem: last table entry with tile0 <= t
 		int lo = 0, hi = nprob - 1;
 		while(lo < hi) {
 			int const mid = (lo + hi + 1) >> 1;
-			if(descs[mid].tile0 <= t) lo = mid;
+			if(tile0[mid] <= t) lo = mid;
 			else hi = mid - 1;
 		}
-		BatchDesc const d = descs[lo];
-		int64_t const item = t - d.tile0;
+		BatchDesc const& d = descs[lo];
+		int64_t const ub = (t - tile0[lo]) * d.per_tile;
+		int64_t const ue = ub + d.per_tile < d.units ? ub + d.per_tile : d.units;
 		if(d.kind == KIND_COL) {
 			switch(d.vec) {
-				case 4:  col_item<float, 4, 1, false>(d.a, d.b, d.c, d.args, item, 0, d.args.K, red); break;
-				case 2:  col_item<float, 2, 1, false>(d.a, d.b, d.c, d.args, item, 0, d.args.K, red); break;
-				default: col_item<float, 1, 1, false>(d.a, d.b, d.c, d.args, item, 0, d.args.K, red); break;
+				case 4:  batch_col_tile<4>(d, ub, ue, red); break;
+				case 2:  batch_col_tile<2>(d, ub, ue, red); break;
+				default: batch_col_tile<1>(d, ub, ue, red); break;
 			}
 		} else {
 			switch(d.vec) {
-				case 4:  batch_dot_tile<4>(d, item); break;
-				case 2:  batch_dot_tile<2>(d, item); break;
-				default: batch_dot_tile<1>(d, item); break;
+				case 4:  batch_dot_tile<4>(d, ub, ue); break;
+				case 2:  batch_dot_tile<2>(d, ub, ue); break;
+				default: batch_dot_tile<1>(d, ub, ue); break;
 			}
 		}
 	}
@@ -97,6 +110,7 @@ struct Batch {
 	int grid = 0;
 	void* dmem = nullptr;              // descriptors followed by the tile counter
 	BatchDesc* descs = nullptr;
+	int64_t* tile0 = nullptr;
 	unsigned int* counter = nullptr;
 	hipStream_t capture_stream = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
@@ -109,8 +123,8 @@ namespace
 int enqueue(Batch& bt, hipStream_t s)
 {
 	hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, bt.counter);
-	hipLaunchKernelGGL(k_batch, dim3(static_cast<unsigned>(bt.grid)), dim3(kBlock), 0, s, bt.descs, bt.nprob,
-	                   bt.total_tiles, bt.counter);
+	hipLaunchKernelGGL(k_batch, dim3(static_cast<unsigned>(bt.grid)), dim3(kBlock), 0, s, bt.descs, bt.tile0,
+	                   bt.nprob, bt.total_tiles, bt.counter);
 	hipError_t e = hipGetLastError();
 	return e == hipSuccess ? TTVX_OK : hip_fail(e, "batch launch");
 }
@@ -135,7 +149,7 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 	struct Item {
 		BatchDesc d;
 		int64_t tiles;
-		int64_t tile_cost;
+		int64_t cost;
 		int index;
 	};
 	std::vector<Item> items;
@@ -157,19 +171,25 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 		Config cfg;
 		Item it;
 		std::memset(&it, 0, sizeof(it));
-		if(configure(V_COL_U1, f, dt, aaddr, di.cus, 0u, cfg)) {
-			it.d.kind = KIND_COL;
-			it.tiles  = cfg.args.items;
-			it.tile_cost = static_cast<int64_t>(cfg.args.OS) * cfg.args.IT * cfg.vec * f.K;
-		} else if(configure(V_DOT_U2, f, dt, aaddr, di.cus, 0u, cfg)) {
-			it.d.kind = KIND_DOT;
-			it.tiles  = (cfg.args.groups + kBlock / 64 - 1) / (kBlock / 64);
-			it.tile_cost = static_cast<int64_t>(kBlock / 64) * (64 / cfg.args.LPR) * kBatchDotU * f.K;
+		int64_t unit_bytes;
+		if(configure(V_COL_U2, f, dt, aaddr, di.cus, 0u, cfg)) {
+			it.d.kind  = KIND_COL;
+			it.d.units = cfg.args.items;
+			unit_bytes = static_cast<int64_t>(cfg.args.OS) * kBatchColU * cfg.args.IT * cfg.vec * f.K * 4;
+		} else if(configure(V_DOT_U4, f, dt, aaddr, di.cus, 0u, cfg)) {
+			it.d.kind  = KIND_DOT;
+			it.d.units = cfg.args.groups;
+			unit_bytes = static_cast<int64_t>(64 / cfg.args.LPR) * kBatchDotU * f.K * 4;
 		} else {
 			return fail(TTVX_ERR_UNSUPPORTED, where + "the batched kernel needs A and C to fold to three dense-inner dimensions");
 		}
-		if(cfg.kind == KIND_COL && cfg.args.red == 2 && cfg.vec > 4)
+		if(cfg.U != (it.d.kind == KIND_COL ? kBatchColU : kBatchDotU) || cfg.vec > 4 ||
+		   cfg.shmem > sizeof(float) * kBlock * 4 * kBatchColU)
 			return fail(TTVX_ERR_UNSUPPORTED, where + "tile does not fit the batched kernel");
+		it.d.per_tile = std::max<int64_t>(1, kTileBytes / std::max<int64_t>(unit_bytes, 1));
+		if(it.d.kind == KIND_DOT) it.d.per_tile = (it.d.per_tile + kBlock / 64 - 1) / (kBlock / 64) * (kBlock / 64);
+		it.tiles = (it.d.units + it.d.per_tile - 1) / it.d.per_tile;
+		it.cost  = f.outputs * f.K;
 		it.d.args = cfg.args;
 		it.d.args.kchunk  = f.K;
 		it.d.args.partial = nullptr;
@@ -184,36 +204,44 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 		items.push_back(it);
 		bytes += f.outputs * f.K * 4 + f.outputs * 4 + f.K * 4;
 	}
-	std::stable_sort(items.begin(), items.end(), [](Item const& x, Item const& y) { return x.tile_cost > y.tile_cost; });
+	std::stable_sort(items.begin(), items.end(), [](Item const& x, Item const& y) { return x.cost > y.cost; });
 
 	auto* bt = new Batch();
 	bt->device = di.id;
 	bt->nprob = static_cast<int>(items.size());
 	bt->bytes = bytes;
 	std::vector<BatchDesc> host(items.size());
+	std::vector<int64_t> tile0(items.size());
 	int64_t tile = 0;
 	for(size_t n = 0; n < items.size(); ++n) {
-		host[n] = items[n].d;
-		host[n].tile0 = tile;
+		host[n]  = items[n].d;
+		tile0[n] = tile;
 		tile += items[n].tiles;
 		bt->order.push_back(items[n].index);
 	}
 	bt->total_tiles = tile;
+	// every block draws once past the end: the counter must not wrap
 	if(tile >= (int64_t{1} << 32) - kBlock * 8192) {
 		delete bt;
 		return fail(TTVX_ERR_UNSUPPORTED, "ttvx_batch_create: too many tiles for one launch");
 	}
-	bt->grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(tile, static_cast<int64_t>(di.cus) * 8)));
-	size_t const dbytes = host.size() * sizeof(BatchDesc);
-	size_t const counter_off = (dbytes + 255) / 256 * 256;     // counter on its own 256-byte line
-	hipError_t e = hipMalloc(&bt->dmem, counter_off + 256);
+	int resident = 0;
+	if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_batch, kBlock, 0) != hipSuccess || resident < 1) resident = 1;
+	bt->grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(tile, static_cast<int64_t>(di.cus) * resident)));
+	size_t const dbytes  = host.size() * sizeof(BatchDesc);
+	size_t const t0_off  = (dbytes + 255) / 256 * 256;
+	size_t const tbytes  = tile0.size() * sizeof(int64_t);
+	size_t const cnt_off = t0_off + (tbytes + 255) / 256 * 256;     // counter on its own 256-byte line
+	hipError_t e = hipMalloc(&bt->dmem, cnt_off + 256);
 	if(e != hipSuccess) {
 		delete bt;
 		return hip_fail(e, "hipMalloc (batch descriptors)");
 	}
-	bt->descs = static_cast<BatchDesc*>(bt->dmem);
-	bt->counter = reinterpret_cast<unsigned int*>(static_cast<char*>(bt->dmem) + counter_off);
+	bt->descs   = static_cast<BatchDesc*>(bt->dmem);
+	bt->tile0   = reinterpret_cast<int64_t*>(static_cast<char*>(bt->dmem) + t0_off);
+	bt->counter = reinterpret_cast<unsigned int*>(static_cast<char*>(bt->dmem) + cnt_off);
 	e = hipMemcpy(bt->descs, host.data(), dbytes, hipMemcpyHostToDevice);
+	if(e == hipSuccess) e = hipMemcpy(bt->tile0, tile0.data(), tbytes, hipMemcpyHostToDevice);
 	if(e != hipSuccess) {
 		(void)hipFree(bt->dmem);
 		delete bt;

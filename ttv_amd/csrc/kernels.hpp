@@ -8,10 +8,12 @@
 //
 //   k_dot  (I == 1, q is the fastest mode; reference gemv_row / dot, matrix_times_vector.h:51-91,
 //           231-295): LPR lanes share a row of K contiguous elements, each lane loads 16-byte
-//           vectors, rows are reduced with cross-lane butterflies.  U rows per lane in flight.
+//           vectors, rows are summed across lanes by recursive halving.  U rows per lane in
+//           flight; rows of one vector per lane are double-buffered across row groups.
 //   k_col  (I > 1; reference gemv_col / gemv_col_blas, matrix_times_vector.h:108-221): IT lanes
 //           along i (16-byte vectors, coalesced), KS lane groups along k, OS output slabs per
-//           item; partial sums over k meet in a butterfly (one wave) or in LDS (several waves).
+//           item; partial sums over k meet by recursive halving (one wave) or in LDS (several
+//           waves).
 //   k_strided  general strides that do not fold to three dimensions: one thread per output.
 //   k_finalize sums the split-n_q partials (FP32 or FP64) and applies FP8 scales.
 //
@@ -22,6 +24,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ttvx
 {
 
@@ -29,6 +33,14 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef float    f32x2 __attribute__((ext_vector_type(2)));
 
+// k unroll of the column tiles (independent loads in flight per lane and slab unit) for 4/8-byte
+// and for 1/2-byte elements
+#ifndef TTVX_KL_WIDE
+#define TTVX_KL_WIDE 8
+#endif
+#ifndef TTVX_KL_NARROW
+#define TTVX_KL_NARROW 4
+#endif
 constexpr int kBlock = 256;   // threads per block for every kernel below (4 waves of 64)
 
 template<typename S> struct Traits;
@@ -137,7 +149,12 @@ template<typename Acc, bool KAHAN> struct Sum;
 template<typename Acc> struct Sum<Acc, false> {
 	Acc s = Acc(0);
 	__device__ __forceinline__ void add(Acc x) { s += x; }
-	__device__ __forceinline__ void fma(Acc a, Acc b) { s = __builtin_fma(a, b, s); }
+	// one fused multiply-add in the accumulation type (the float form must not promote to double)
+	__device__ __forceinline__ void fma(Acc a, Acc b)
+	{
+		if constexpr(std::is_same_v<Acc, float>) s = __builtin_fmaf(a, b, s);
+		else s = __builtin_fma(a, b, s);
+	}
 	__device__ __forceinline__ Acc value() const { return s; }
 };
 template<typename Acc> struct Sum<Acc, true> {
@@ -159,7 +176,7 @@ struct Args {
 	int64_t sCo;              // C stride of o (the i stride is 1)
 	// k_col tiling
 	int IT, KS, OS;           // lanes along i (vectors), lane groups along k, slabs per item
-	int red;                  // cross-lane reduction: 0 none, 1 butterfly in one wave, 2 LDS
+	int red;                  // cross-lane reduction: 0 none, 1 shuffles in one wave, 2 LDS
 	int along_i;              // the U units of a thread lie along i (wide rows) instead of along o
 	int64_t iTiles;           // tiles along i (IT*VEC columns, or U*IT*VEC when along_i)
 	int64_t items;            // ceil(O / (OS*U)) * iTiles, or ceil(O / OS) * iTiles when along_i
@@ -175,7 +192,24 @@ struct Args {
 	// FP8 scales (null for other types)
 	const float* scales;
 	int64_t fps;              // fibers per scale
+	// work assignment: 0 = grid-stride over items/groups, 1 = each block one contiguous range
+	int contig;
 };
+
+// Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
+__device__ __forceinline__ void block_range(int contig, int64_t n, int64_t& begin, int64_t& end, int64_t& step)
+{
+	if(contig) {
+		int64_t const per = (n + gridDim.x - 1) / gridDim.x;
+		begin = static_cast<int64_t>(blockIdx.x) * per;
+		end   = begin + per < n ? begin + per : n;
+		step  = 1;
+	} else {
+		begin = blockIdx.x;
+		end   = n;
+		step  = gridDim.x;
+	}
+}
 
 template<typename Acc>
 __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, int64_t o, int64_t i, Acc v)
@@ -189,6 +223,40 @@ __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, i
 	}
 	if(P.scales != nullptr) v *= static_cast<Acc>(P.scales[j / P.fps]);
 	static_cast<Acc*>(C)[o * P.sCo + i] = v;
+}
+
+// ---- cross-lane sums ------------------------------------------------------------------------
+// A lane holds partial sums v[0..N) of N outputs; the lanes l ^ (m*stride), m < width (stride and
+// width powers of two), hold partial sums of the same N outputs.  Recursive halving: at every
+// exchange step a lane keeps half of its outputs and adds its partner's half of them, so N outputs
+// take N-1 (+ log2(width/N)) shuffles instead of N*log2(width).  On return the lane holds the
+// full sums of outputs base .. base+n-1 in v[0..n), n = max(1, N/width); when width > N, the
+// width/N lanes that differ only in the low bits hold the same sums, and the one with those bits
+// zero is the lane that stores them (the function returns whether this lane stores).
+template<typename Acc, int N>
+__device__ __forceinline__ bool halving_sum(Acc (&v)[N], int lane, int stride, int width, int& base, int& n)
+{
+	base = 0;
+	n = N;
+	int d = width >> 1;
+#pragma unroll
+	for(int h = N / 2; h >= 1; h >>= 1) {
+		if(d >= 1) {
+			bool const up = (lane & (d * stride)) != 0;
+#pragma unroll
+			for(int j = 0; j < h; ++j) {
+				Acc const send = up ? v[j] : v[j + h];
+				Acc const keep = up ? v[j + h] : v[j];
+				v[j] = keep + __shfl_xor(send, d * stride, 64);
+			}
+			if(up) base += h;
+			n = h;
+			d >>= 1;
+		}
+	}
+	bool const store = d < 1 || ((lane / stride) & ((d << 1) - 1)) == 0;
+	for(; d >= 1; d >>= 1) v[0] += __shfl_xor(v[0], d * stride, 64);
+	return store;
 }
 
 // ---- column tiles -------------------------------------------------------------------------
@@ -216,7 +284,7 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 	using Acc = typename Traits<S>::acc;
 	// k unroll: U*KU independent 16-byte loads in flight per lane (8 for 4- and 8-byte elements,
 	// 4 for the narrow types, whose VEC accumulators already take the registers)
-	constexpr int KL = sizeof(S) >= 4 ? 8 : 4;
+	constexpr int KL = sizeof(S) >= 4 ? TTVX_KL_WIDE : TTVX_KL_NARROW;
 	constexpr int KU = KL / U > 1 ? KL / U : 1;
 	int const t  = static_cast<int>(threadIdx.x);
 	int const it = t % P.IT;
@@ -284,12 +352,26 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 		for(int e = 0; e < VEC; ++e) v[u][e] = acc[u][e].value();
 
 	if(P.red == 1) {
-		// lanes it + IT*ks of one wave hold the same outputs
-		for(int d = P.IT; d < P.IT * P.KS; d <<= 1)
+		// lanes it + IT*ks of one wave hold partial sums of the same U*VEC outputs
+		Acc w[U * VEC];
 #pragma unroll
-			for(int u = 0; u < U; ++u)
+		for(int u = 0; u < U; ++u)
 #pragma unroll
-				for(int e = 0; e < VEC; ++e) v[u][e] += __shfl_xor(v[u][e], d, 64);
+			for(int e = 0; e < VEC; ++e) w[u * VEC + e] = v[u][e];
+		int base, n;
+		bool const st = halving_sum<Acc, U * VEC>(w, t & 63, P.IT, P.KS, base, n);
+		if(active && st) {
+#pragma unroll
+			for(int j = 0; j < U * VEC; ++j)
+				if(j < n) {
+					int const f = base + j;
+					int64_t oo, ii;
+					col_unit(P, og, itile, os, it, f / VEC, U, VEC, oo, ii);
+					ii += f % VEC;
+					if(oo < P.O && ii < P.I) store_out<Acc>(C, P, oo, ii, w[j]);
+				}
+		}
+		return;
 	} else if(P.red == 2) {
 		// partial sums meet in LDS; every thread then finishes some of the item's outputs
 		// (output r = ((os*U + u)*IT + it)*VEC + e, partials added in ascending ks)
@@ -336,13 +418,16 @@ __global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const t
 	Acc* red = reinterpret_cast<Acc*>(smem_raw);
 	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
 	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
-	for(int64_t item = blockIdx.x; item < P.items; item += gridDim.x)
+	int64_t ib, ie, is;
+	block_range(P.contig, P.items, ib, ie, is);
+	for(int64_t item = ib; item < ie; item += is)
 		col_item<S, VEC, U, KAHAN>(A, b, C, P, item, kb, ke, red);
 }
 
 // ---- row tiles (q fastest) ----------------------------------------------------------------
 // A wave handles a row group g: U*RPW rows (RPW = 64/LPR).  Lane l: row r = l / LPR inside each
-// of the U row sets, k offset (l % LPR)*VEC.  Rows are reduced with butterflies over LPR lanes.
+// of the U row sets, k offset (l % LPR)*VEC.  Rows are summed over their LPR lanes by recursive
+// halving, after which the U*RPW sums of a group are stored by one instruction (consecutive rows).
 // vector steps of b a lane keeps in registers for short rows: 16 values of b whatever the type
 __host__ __device__ constexpr int dot_breg(int vec) { return vec >= 16 ? 1 : (vec >= 8 ? 2 : 4); }
 
@@ -363,6 +448,79 @@ __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __rest
 		}
 }
 
+// sums the lanes' partial row sums of group g and stores the U*RPW rows
+template<typename Acc, int U>
+__device__ __forceinline__ void dot_store(void* __restrict__ C, const Args& P, int64_t g, Acc (&v)[U])
+{
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	int const r    = lane >> P.lprShift;
+	int const rpw  = 64 >> P.lprShift;
+	int base, n;
+	if(halving_sum<Acc, U>(v, lane, 1, P.LPR, base, n)) {
+#pragma unroll
+		for(int j = 0; j < U; ++j)
+			if(j < n) {
+				int64_t const row = (g * U + base + j) * rpw + r;
+				if(row < P.O) store_out<Acc>(C, P, row, 0, v[j]);
+			}
+	}
+}
+
+// pointer to the start of this lane's piece of row set u of group g (rows past the end clamped:
+// those lanes re-read a live row and store nothing)
+template<typename S, int U>
+__device__ __forceinline__ const S* dot_row_ptr(const S* __restrict__ A, const Args& P, int64_t g, int u, int r, int64_t k0)
+{
+	int64_t const row = (g * U + u) * (64 >> P.lprShift) + r;
+	int64_t const rc  = row < P.O ? row : P.O - 1;
+	return A + rc * P.sAo + k0;
+}
+
+// rows of at most LPR*VEC elements (one vector per lane and row): double-buffered, the next
+// group's vectors are in flight while the current group is summed and stored
+template<typename S, int VEC, int U, bool KAHAN>
+__device__ __forceinline__ void dot_rows_pipelined(const S* __restrict__ A, void* __restrict__ C, const Args& P,
+                                                   int64_t g0, int64_t ge, int64_t gs, int64_t kb, int64_t ke,
+                                                   const typename Traits<S>::acc (&bb)[VEC])
+{
+	using Acc = typename Traits<S>::acc;
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	int const r    = lane >> P.lprShift;
+	int64_t const k0 = kb + static_cast<int64_t>(lane & (P.LPR - 1)) * VEC;
+	bool const live = k0 < ke;      // lanes past the end of the row contribute zero
+	auto load = [&](int64_t g, Vec<S, VEC> (&raw)[U]) {
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			if(live) raw[u].load(dot_row_ptr<S, U>(A, P, g, u, r, k0));
+			else
+#pragma unroll
+				for(int w = 0; w < Vec<S, VEC>::words; ++w) raw[u].w[w] = 0u;
+		}
+	};
+	auto finish = [&](int64_t g, const Vec<S, VEC> (&raw)[U]) {
+		Acc v[U];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			Sum<Acc, KAHAN> acc;
+			Acc x[VEC];
+			decode(raw[u], x);
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
+			v[u] = acc.value();
+		}
+		dot_store<Acc, U>(C, P, g, v);
+	};
+	Vec<S, VEC> ra[U], rb[U];
+	if(g0 < ge) load(g0, ra);
+	for(int64_t g = g0; g < ge; g += 2 * gs) {
+		bool const has1 = g + gs < ge;
+		if(has1) load(g + gs, rb);
+		finish(g, ra);
+		if(g + 2 * gs < ge) load(g + 2 * gs, ra);
+		if(has1) finish(g + gs, rb);
+	}
+}
+
 // short rows: every k step of the lane uses b from registers (P.kiter <= dot_breg(VEC))
 template<typename S, int VEC, int U, bool KAHAN>
 __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* __restrict__ C, const Args& P, int64_t g,
@@ -373,17 +531,11 @@ __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* _
 	int const lane = static_cast<int>(threadIdx.x) & 63;
 	int const r    = lane >> P.lprShift;
 	int const kl   = lane & (P.LPR - 1);
-	int const rpw  = 64 >> P.lprShift;
 	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
 	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
-	int64_t row[U];
 	const S* pa[U];
 #pragma unroll
-	for(int u = 0; u < U; ++u) {
-		row[u] = (g * U + u) * rpw + r;
-		int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
-		pa[u] = A + rc * P.sAo + k0;
-	}
+	for(int u = 0; u < U; ++u) pa[u] = dot_row_ptr<S, U>(A, P, g, u, r, k0);
 	Sum<Acc, KAHAN> acc[U];
 #pragma unroll
 	for(int s = 0; s < dot_breg(VEC); ++s) {
@@ -403,14 +555,7 @@ __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* _
 	Acc v[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-	for(int d = 1; d < P.LPR; d <<= 1)
-#pragma unroll
-		for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
-	if(kl == 0) {
-#pragma unroll
-		for(int u = 0; u < U; ++u)
-			if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
-	}
+	dot_store<Acc, U>(C, P, g, v);
 }
 
 // long rows: b is re-read through the cache, loads of U rows and two k steps overlap
@@ -422,17 +567,11 @@ __device__ __forceinline__ void dot_group_long(const S* __restrict__ A, const ty
 	int const lane = static_cast<int>(threadIdx.x) & 63;
 	int const r    = lane >> P.lprShift;
 	int const kl   = lane & (P.LPR - 1);
-	int const rpw  = 64 >> P.lprShift;
 	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
 	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
-	int64_t row[U];
 	const S* pa[U];
 #pragma unroll
-	for(int u = 0; u < U; ++u) {
-		row[u] = (g * U + u) * rpw + r;
-		int64_t const rc = row[u] < P.O ? row[u] : P.O - 1;
-		pa[u] = A + rc * P.sAo;
-	}
+	for(int u = 0; u < U; ++u) pa[u] = dot_row_ptr<S, U>(A, P, g, u, r, 0);
 	Sum<Acc, KAHAN> acc[U];
 	int64_t k = k0;
 	for(; k + kstride < ke; k += 2 * kstride) {
@@ -473,14 +612,7 @@ __device__ __forceinline__ void dot_group_long(const S* __restrict__ A, const ty
 	Acc v[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-	for(int d = 1; d < P.LPR; d <<= 1)
-#pragma unroll
-		for(int u = 0; u < U; ++u) v[u] += __shfl_xor(v[u], d, 64);
-	if(kl == 0) {
-#pragma unroll
-		for(int u = 0; u < U; ++u)
-			if(row[u] < P.O) store_out<Acc>(C, P, row[u], 0, v[u]);
-	}
+	dot_store<Acc, U>(C, P, g, v);
 }
 
 template<typename S, int VEC, int U, bool KAHAN>
@@ -490,14 +622,27 @@ __global__ __launch_bounds__(kBlock) void k_dot(const S* __restrict__ A, const t
 	using Acc = typename Traits<S>::acc;
 	int64_t const kb = static_cast<int64_t>(blockIdx.y) * P.kchunk;
 	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
-	int64_t const wave  = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
-	int64_t const waves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+	// groups: contiguous per block (the block's waves interleaved) or grid-stride over all waves
+	int64_t g0, ge, gs;
+	if(P.contig) {
+		block_range(1, P.groups, g0, ge, gs);
+		g0 += threadIdx.x >> 6;
+		gs = kBlock / 64;
+	} else {
+		g0 = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+		ge = P.groups;
+		gs = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+	}
 	if(P.kiter <= dot_breg(VEC)) {
 		Acc bb[dot_breg(VEC)][VEC];
 		dot_load_b<S, VEC>(b, P, kb, ke, bb);
-		for(int64_t g = wave; g < P.groups; g += waves) dot_group_short<S, VEC, U, KAHAN>(A, C, P, g, kb, ke, bb);
+		if(P.kiter == 1) {
+			dot_rows_pipelined<S, VEC, U, KAHAN>(A, C, P, g0, ge, gs, kb, ke, bb[0]);
+		} else {
+			for(int64_t g = g0; g < ge; g += gs) dot_group_short<S, VEC, U, KAHAN>(A, C, P, g, kb, ke, bb);
+		}
 	} else {
-		for(int64_t g = wave; g < P.groups; g += waves) dot_group_long<S, VEC, U, KAHAN>(A, b, C, P, g, kb, ke);
+		for(int64_t g = g0; g < ge; g += gs) dot_group_long<S, VEC, U, KAHAN>(A, b, C, P, g, kb, ke);
 	}
 }
 

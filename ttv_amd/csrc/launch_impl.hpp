@@ -3,29 +3,45 @@
 // compile in parallel.
 #pragma once
 
+#include <algorithm>
+
 #include "runtime.hpp"
 
 namespace ttvx
 {
 
+// Launches kernel with the configuration's grid, or, for contiguous schedules, with as many
+// blocks as the GPU holds at once (the kernel's occupancy), so every block owns one range.
+template<typename Kern, typename... Xs>
+void launch_sched(Kern kernel, const Config& cfg, hipStream_t s, Xs... xs)
+{
+	unsigned gx = static_cast<unsigned>(cfg.grid);
+	if(cfg.args.contig) {
+		int nb = 0;
+		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, cfg.shmem) != hipSuccess || nb < 1) nb = 1;
+		int64_t const work = cfg.kind == KIND_DOT ? (cfg.args.groups + kBlock / 64 - 1) / (kBlock / 64) : cfg.args.items;
+		int64_t const resident = static_cast<int64_t>(cfg.cus) * nb;
+		gx = static_cast<unsigned>(std::max<int64_t>(1, std::min(work, resident)));
+	}
+	hipLaunchKernelGGL(kernel, dim3(gx, static_cast<unsigned>(cfg.ksplit)), dim3(kBlock), cfg.shmem, s, xs...);
+}
+
 template<typename S, int VEC, bool KH>
 int launch_vec(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
 {
-	dim3 const grid(static_cast<unsigned>(cfg.grid), static_cast<unsigned>(cfg.ksplit));
-	dim3 const block(kBlock);
 	if(cfg.kind == KIND_DOT) {
 		switch(cfg.U) {
-			case 1: hipLaunchKernelGGL((k_dot<S, VEC, 1, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
-			case 2: hipLaunchKernelGGL((k_dot<S, VEC, 2, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
-			case 4: hipLaunchKernelGGL((k_dot<S, VEC, 4, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
-			case 8: hipLaunchKernelGGL((k_dot<S, VEC, 8, KH>), grid, block, 0, s, A, B, C, cfg.args); break;
+			case 1: launch_sched(k_dot<S, VEC, 1, KH>, cfg, s, A, B, C, cfg.args); break;
+			case 2: launch_sched(k_dot<S, VEC, 2, KH>, cfg, s, A, B, C, cfg.args); break;
+			case 4: launch_sched(k_dot<S, VEC, 4, KH>, cfg, s, A, B, C, cfg.args); break;
+			case 8: launch_sched(k_dot<S, VEC, 8, KH>, cfg, s, A, B, C, cfg.args); break;
 			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported row unroll");
 		}
 	} else {
 		switch(cfg.U) {
-			case 1: hipLaunchKernelGGL((k_col<S, VEC, 1, KH>), grid, block, cfg.shmem, s, A, B, C, cfg.args); break;
-			case 2: hipLaunchKernelGGL((k_col<S, VEC, 2, KH>), grid, block, cfg.shmem, s, A, B, C, cfg.args); break;
-			case 4: hipLaunchKernelGGL((k_col<S, VEC, 4, KH>), grid, block, cfg.shmem, s, A, B, C, cfg.args); break;
+			case 1: launch_sched(k_col<S, VEC, 1, KH>, cfg, s, A, B, C, cfg.args); break;
+			case 2: launch_sched(k_col<S, VEC, 2, KH>, cfg, s, A, B, C, cfg.args); break;
+			case 4: launch_sched(k_col<S, VEC, 4, KH>, cfg, s, A, B, C, cfg.args); break;
 			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported slab unroll");
 		}
 	}

@@ -147,6 +147,9 @@ int pick_vec(const DType& dt, uintptr_t a_addr, int64_t inner, std::initializer_
 
 std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 
+#ifndef TTVX_CONTIG_DEFAULT
+#define TTVX_CONTIG_DEFAULT 1   // contiguous block ranges over the resident grid
+#endif
 constexpr int kGridPerCU = 16;   // resident-block cap of the grid-stride loops
 } // namespace
 
@@ -154,6 +157,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 {
 	cfg = Config{};
 	cfg.variant = v;
+	cfg.cus = std::max(cus, 1);
 	Args& P = cfg.args;
 	std::memset(&P, 0, sizeof(P));
 	P.fps = 1;
@@ -260,6 +264,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		return false;
 	}
 
+	P.contig = TTVX_CONTIG_DEFAULT;
 	if(cfg.kind == KIND_COL) {
 		if(P.KS == 1) P.red = 0;
 		else if(is_pow2(P.IT) && is_pow2(P.KS) && P.IT * P.KS <= 64) P.red = 1;
@@ -281,6 +286,7 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 		if(configure(V_DOT_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_DOT_SPLIT;
 		configure(V_DOT_U1, f, dt, a_addr, cus, flags, cfg);
 		int64_t const kiter = cfg.args.kiter;
+		if(dt.size < 4) return kiter <= 2 ? V_DOT_U4 : V_DOT_U2;   // 8/16 decoded values per vector
 		if(kiter <= 1) return V_DOT_U8;
 		if(kiter <= 2) return V_DOT_U4;
 		if(kiter <= 4) return V_DOT_U2;
@@ -289,6 +295,9 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 	if(f.I > 1 && f.sAi == 1 && f.sCi == 1) {
 		if(configure(V_COL_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_COL_SPLIT;
 		configure(V_COL_U1, f, dt, a_addr, cus, flags, cfg);
+		// narrow types: a 16-byte vector already holds 8 (bf16) or 16 (fp8) columns per lane
+		if(dt.size == 2) return V_COL_U2;
+		if(dt.size == 1) return V_COL_NARROW;
 		int64_t const iv = f.I / cfg.vec;
 		if(iv >= 4 * kBlock) return V_COL_U4;            // wide rows: 4 vectors per lane along i
 		int64_t const per_thread = f.K / std::max(1, cfg.args.KS);

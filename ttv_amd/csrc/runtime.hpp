@@ -62,7 +62,8 @@ struct Config {
 	Args args{};
 	StridedArgs sargs{};
 	int ksplit = 1;
-	int grid = 1;
+	int grid = 1;          // work-bound grid; contiguous schedules shrink it to the resident blocks
+	int cus = 1;
 	std::size_t shmem = 0;
 };
 
