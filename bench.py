@@ -42,6 +42,35 @@ def case_bytes(nq, elem=4, c_elem=4, extra=0):
     return N * elem + (N // nq) * c_elem + nq * 4 + extra
 
 
+def spin(torch, ttv_amd, us, sink):
+    """Holds the GPU stream for about `us` microseconds, so that the launches enqueued after it are
+    all queued before the GPU reaches them: no launch then waits for the host, and event pairs
+    time the kernels alone."""
+    import ctypes
+    iters = int(min(100000, max(1, us / 3.4)))
+    ttv_amd.check(ttv_amd.lib().ttvx_bench_spin(iters, ctypes.c_void_p(sink.data_ptr()),
+                                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def timed_launches(torch, ttv_amd, run, reps, sink, flush=None, us_per_launch=150.0):
+    """Times `reps` launches of run() with HIP events on the current stream, all enqueued behind a
+    spin (and, with flush, each behind a write of the flush buffer that evicts the last-level cache)."""
+    run()
+    torch.cuda.synchronize()
+    spin(torch, ttv_amd, reps * (us_per_launch + (100.0 if flush is not None else 0.0)), sink)
+    evs = []
+    for _ in range(reps):
+        if flush is not None:
+            flush.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in evs]
+
+
 def traffic_from_profiles():
     """HBM bytes per launch of the dominant kernel measured with rocprofv3 counters (committed
     summary), or None."""
@@ -52,6 +81,82 @@ def traffic_from_profiles():
     return None
 
 
+CASE_ORDER = [(lname, q) for lname in ("first", "last") for q in range(1, 5)]
+
+
+def summarize_profiles(d, traffic_out=None):
+    """Summary text of the three rocprofv3 runs of `bench.py --profile` under directory d
+    (prof_trace: --kernel-trace --stats; prof_fetch: --pmc FETCH_SIZE; prof_write: --pmc WRITE_SIZE)
+    and, with traffic_out, the per-case HBM traffic JSON that the benchmark line reports.
+    Counters are calibrated with the plain copy kernel of the same sequence (1 GiB read, 1 GiB
+    written per launch)."""
+    import csv
+    import re
+
+    def short(n):
+        return re.sub(r"\(.*", "", n).replace("void ", "")
+
+    def rows(path):
+        with open(path) as f:
+            return list(csv.DictReader(f))
+
+    out = []
+    stats = rows(os.path.join(d, "prof_trace", "run_kernel_stats.csv"))
+    out.append("Kernel statistics (run_kernel_stats.csv):")
+    for r in stats:
+        if "ttvx" in r["Name"]:
+            out.append("  %-58s calls=%3s avg_us=%8.1f min_us=%8.1f max_us=%8.1f" % (
+                short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
+                float(r["MaxNs"]) / 1e3))
+    def after_streams(rs):
+        """ttvx dispatches in dispatch order; the products timed are those after the stream kernels
+        (the ones before them only allocated the outputs)"""
+        rs = sorted((r for r in rs if "ttvx" in r["Kernel_Name"]), key=lambda r: int(r["Dispatch_Id"]))
+        last = max(i for i, r in enumerate(rs) if "stream" in r["Kernel_Name"])
+        return rs, [r for r in rs[last + 1:] if "stream" not in r["Kernel_Name"]]
+
+    _, prod_t = after_streams(rows(os.path.join(d, "prof_trace", "run_kernel_trace.csv")))
+
+    def counters(name):
+        rs, prod = after_streams(rows(os.path.join(d, "prof_" + name, "run_counter_collection.csv")))
+        copy = [float(r["Counter_Value"]) for r in rs if "k_copy_stream" in r["Kernel_Name"]]
+        return sum(copy) / len(copy), [(short(r["Kernel_Name"]), float(r["Counter_Value"])) for r in prod]
+
+    fetch_copy, fetch = counters("fetch")
+    write_copy, write = counters("write")
+    gib_kib = float(1 << 20)
+    fscale, wscale = gib_kib / fetch_copy, gib_kib / write_copy
+    out.append("")
+    out.append("Counter calibration with the plain copy of 1 GiB (reads 1,048,576 KiB, writes 1,048,576 KiB):")
+    out.append("  FETCH_SIZE read %.1f KiB -> scale %.4f ; WRITE_SIZE read %.1f KiB -> scale %.4f" % (
+        fetch_copy, fscale, write_copy, wscale))
+    n_case = len(CASE_ORDER)
+    reps = len(prod_t) // n_case
+    assert reps >= 1 and len(prod_t) == reps * n_case and len(fetch) == len(prod_t) and len(write) == len(prod_t), \
+        "expected %d product dispatches per run" % (reps * n_case)
+    alg = N * 4 + (N // 128) * 4 + 128 * 4
+    cases = []
+    out.append("")
+    out.append("Per case (mean of %d launches): trace duration, HBM traffic from the scaled counters, vs %d "
+               "algorithmic bytes:" % (reps, alg))
+    for i, (lname, q) in enumerate(CASE_ORDER):
+        sl = slice(i * reps, (i + 1) * reps)
+        us = sum((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in prod_t[sl]) / reps
+        fb = sum(v for _, v in fetch[sl]) / reps * 1024 * fscale
+        wb = sum(v for _, v in write[sl]) / reps * 1024 * wscale
+        name = short(prod_t[i * reps]["Kernel_Name"])
+        cases.append(dict(layout=lname, q=q, kernel=name, avg_us=round(us, 2), fetch_bytes=round(fb),
+                          write_bytes=round(wb), hbm_bytes_per_launch=round(fb + wb),
+                          traffic_over_algorithmic=round((fb + wb) / alg, 4)))
+        out.append("  %-5s q=%d %-36s %7.1f us  fetch %.4e B  write %.4e B  (%.4f x algorithmic, %.1f GB/s)" % (
+            lname, q, name.replace("ttvx::", ""), us, fb, wb, (fb + wb) / alg, alg / (us * 1e-6) / 1e9))
+    if traffic_out:
+        with open(traffic_out, "w") as f:
+            json.dump(dict(workload="FP32 128^4, q=1..4 x {first,last}-order", algorithmic_bytes=alg,
+                           fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), cases=cases), f, indent=1)
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,11 +165,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 16^7 FP32 and 256^3 FP64 cases")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile", action="store_true",
                     help="short fixed dispatch sequence for rocprofv3: copy and read streams, then each of the "
                          "8 products 3 times; prints no JSON")
+    ap.add_argument("--summarize", metavar="DIR",
+                    help="print the summary of the rocprofv3 runs of --profile under DIR (no GPU needed)")
+    ap.add_argument("--traffic-out", metavar="JSON", help="with --summarize: write the per-case traffic here")
     args = ap.parse_args()
+    if args.summarize:
+        print(summarize_profiles(args.summarize, args.traffic_out))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are out of scope (the mode-q product "
               "shards over free modes with no exchange, see DESIGN.md)", file=sys.stderr)
@@ -125,7 +237,9 @@ def main():
           for _ in range(args.steps)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
+    sink = torch.zeros(1, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
+    spin(torch, ttv_amd, args.steps * len(cases) * 80.0, sink)     # the host enqueues the whole region
     t_start.record(stream)
     for s in range(args.steps):
         for i, c in enumerate(cases):
@@ -149,10 +263,12 @@ def main():
                                                                        pc["gbs"], pc["frac"]))
     dom = max(cases, key=lambda c: c["ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3)
-    traffic = traffic_from_profiles()
+    traffic = None
+    for t in (traffic_from_profiles() or {}).get("cases", []):
+        if t["layout"] == dom["layout"] and t["q"] == dom["q"]:
+            traffic = t["hbm_bytes_per_launch"]
     roofline = dict(bound="hbm", achieved=round(achieved / 1e9, 1), peak=8000.0, unit="GB/s",
-                    frac=round(achieved / PEAK_BPS, 4),
-                    traffic=(traffic or {}).get("hbm_bytes_per_launch"),
+                    frac=round(achieved / PEAK_BPS, 4), traffic=traffic,
                     kernel="%s (%s-order, q=%d)" % (dom["kernel"], dom["layout"], dom["q"]),
                     algorithmic_bytes_per_launch=dom["bytes"], launch_ms=round(dom["ms"], 4))
 
@@ -162,15 +278,8 @@ def main():
     for _ in range(3):
         ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
                                         stream.cuda_stream))
-    rts = []
-    for _ in range(10):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
-                                        stream.cuda_stream))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        rts.append(e0.elapsed_time(e1))
+    rts = timed_launches(torch, ttv_amd, lambda: ttv_amd.check(L.ttvx_bench_read(
+        Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(), stream.cuda_stream)), 10, sink)
     read_gbs = N * 4 / (np.median(rts) * 1e-3) / 1e9
     log("read stream ceiling: %.1f GB/s (%.3f of peak)" % (read_gbs, read_gbs * 1e9 / PEAK_BPS))
 
@@ -212,7 +321,54 @@ def main():
         del a_host
         out["batching"] = batching_leg(torch, np, ttv_amd, log)
 
+    if not args.no_extra:
+        out["other_configs"] = other_configs_leg(torch, np, ttv_amd, Aflat, log)
+
     print(json.dumps(out), flush=True)
+
+
+def other_configs_leg(torch, np, ttv_amd, Aflat, log, reps=5):
+    """The other shapes of the reference's benchmark: FP32 16^7 (2^28 elements, every q, first- and
+    last-order layout; same data as 128^4) and FP64 256^3 first-order with q = 2.  The FP64 tensor
+    (128 MiB) fits in the 256 MB last-level cache, so a 512 MiB buffer is written before every
+    timed launch."""
+    from ttv_amd.meta import generate_strides
+    dev = Aflat.device
+    flush = torch.empty(1 << 27, dtype=torch.float32, device=dev)
+    sink = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def time_one(run, nbytes):
+        ms = float(np.median(timed_launches(torch, ttv_amd, run, reps, sink, flush=flush)))
+        return ms, nbytes / (ms * 1e-3) / 1e9
+
+    res = {"fp32_16e7": [], "fp64_256e3": None}
+    sh = [16] * 7
+    for lname, pia in (("first", list(range(1, 8))), ("last", list(range(7, 0, -1)))):
+        wa = generate_strides(sh, pia)
+        A = Aflat.as_strided(sh, wa)
+        for q in range(1, 8):
+            b = torch.rand(16, device=dev)
+            C = ttv_amd.ttv(q, A, b)
+            info = ttv_amd.plan_info(q, sh, wa, pia)
+            nbytes = N * 4 + (N // 16) * 4 + 16 * 4
+            ms, gbs = time_one(lambda: ttv_amd.ttv(q, A, b, out=C), nbytes)
+            res["fp32_16e7"].append(dict(layout=lname, q=q, kernel=ttv_amd.variant_names()[info["variant"]],
+                                         ms=round(ms, 4), gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4)))
+            log("  16^7 %-5s q=%d %-10s %.4f ms %7.1f GB/s" % (lname, q, res["fp32_16e7"][-1]["kernel"], ms, gbs))
+    n3 = 1 << 24
+    A64 = Aflat[:n3].double().view(256, 256, 256).permute(2, 1, 0)     # first-order strides (1, 256, 65536)
+    b64 = torch.rand(256, device=dev, dtype=torch.float64)
+    C64 = ttv_amd.ttv(2, A64, b64)
+    nbytes = n3 * 8 + 65536 * 8 + 256 * 8
+    ms, gbs = time_one(lambda: ttv_amd.ttv(2, A64, b64, out=C64), nbytes)
+    info = ttv_amd.plan_info(2, [256] * 3, list(A64.stride()), [1, 2, 3], ttv_amd.F64)
+    res["fp64_256e3"] = dict(q=2, layout="first", kernel=ttv_amd.variant_names()[info["variant"]], ms=round(ms, 4),
+                             gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4))
+    log("  256^3 FP64 q=2 %.4f ms %7.1f GB/s" % (ms, gbs))
+    fr = [c["frac"] for c in res["fp32_16e7"]]
+    res["fp32_16e7_min_frac"] = min(fr)
+    res["fp32_16e7_mean_frac"] = round(float(np.mean(fr)), 4)
+    return res
 
 
 def batching_leg(torch, np, ttv_amd, log, count=512, seed=4):
@@ -278,6 +434,8 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
     from oracle import oracle as O
     stream = torch.cuda.current_stream()
     res = {"bf16": [], "fp8": []}
+    sink = torch.zeros(1, dtype=torch.int32, device=Aflat.device)
+    flush = torch.empty(1 << 27, dtype=torch.float32, device=Aflat.device)   # 512 MiB > last-level cache
     Abf = ttv_amd.quantize_bf16(Aflat)            # same strides (flat)
     abf_host = None
     for c in cases:
@@ -301,16 +459,7 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
                 run = lambda: ttv_amd.ttv_fp8(q, codes, scales, fps, b, out=Cq)  # noqa: E731
                 extra = scales.numel() * 4
                 elem = 1
-            for _ in range(2):
-                run()
-            ts = []
-            for _ in range(max(5, args.steps)):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                run()
-                e1.record(stream)
-                torch.cuda.synchronize()
-                ts.append(e0.elapsed_time(e1))
+            ts = timed_launches(torch, ttv_amd, run, max(5, args.steps), sink, flush=flush)
             ms = float(np.mean(ts))
             got = Cq.as_strided([Cq.numel()], [1]).cpu().numpy().astype(np.float64)
             err = float(np.linalg.norm(got - ref) / rn)

@@ -39,7 +39,7 @@ typedef float    f32x2 __attribute__((ext_vector_type(2)));
 #define TTVX_KL_WIDE 8
 #endif
 #ifndef TTVX_KL_NARROW
-#define TTVX_KL_NARROW 4
+#define TTVX_KL_NARROW 8
 #endif
 constexpr int kBlock = 256;   // threads per block for every kernel below (4 waves of 64)
 
@@ -55,32 +55,35 @@ template<> struct Traits<uint8_t>  { using acc = float;  static constexpr int vm
 
 // A is streamed exactly once, so its loads are non-temporal (no reuse to keep in the caches);
 // measured on MI355X this lifts a pure 1 GiB read stream from ~6.0 to ~7.0 TB/s.
+// A is device memory: the loads go through the global (not the flat) path, so they only count
+// against vmcnt.
+#define TTVX_GLOBAL(T, p) ((const __attribute__((address_space(1))) T*)(p))
 template<int BYTES> struct RawLoad;
 template<> struct RawLoad<16> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[4]) {
-		u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+		u32x4 v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x4, p));
 		w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
 	}
 };
 template<> struct RawLoad<8> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[2]) {
-		u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+		u32x2 v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x2, p));
 		w[0] = v.x; w[1] = v.y;
 	}
 };
 template<> struct RawLoad<4> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p));
+		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(uint32_t, p));
 	}
 };
 template<> struct RawLoad<2> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(uint16_t, p));
 	}
 };
 template<> struct RawLoad<1> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = __builtin_nontemporal_load(reinterpret_cast<const uint8_t*>(p));
+		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(uint8_t, p));
 	}
 };
 
@@ -318,6 +321,9 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 			for(int j = 0; j < KU; ++j)
 #pragma unroll
 				for(int u = 0; u < U; ++u) raw[j][u].load(pa[u] + j * kstep);
+			// every load of the step is issued before any of them is used: without this fence the
+			// scheduler may trade loads in flight for registers and serialise them
+			__builtin_amdgcn_sched_barrier(0);
 #pragma unroll
 			for(int j = 0; j < KU; ++j)
 #pragma unroll

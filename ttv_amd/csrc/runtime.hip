@@ -147,8 +147,10 @@ int pick_vec(const DType& dt, uintptr_t a_addr, int64_t inner, std::initializer_
 
 std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 
+// Work assignment of k_dot / k_col: 0 = grid-stride over up to kGridPerCU blocks per CU (measured
+// faster on MI355X for rows and narrow column tiles), 1 = one contiguous range per resident block.
 #ifndef TTVX_CONTIG_DEFAULT
-#define TTVX_CONTIG_DEFAULT 1   // contiguous block ranges over the resident grid
+#define TTVX_CONTIG_DEFAULT 0
 #endif
 constexpr int kGridPerCU = 16;   // resident-block cap of the grid-stride loops
 } // namespace
@@ -286,7 +288,7 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 		if(configure(V_DOT_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_DOT_SPLIT;
 		configure(V_DOT_U1, f, dt, a_addr, cus, flags, cfg);
 		int64_t const kiter = cfg.args.kiter;
-		if(dt.size < 4) return kiter <= 2 ? V_DOT_U4 : V_DOT_U2;   // 8/16 decoded values per vector
+		if(dt.size == 1) return kiter <= 2 ? V_DOT_U4 : V_DOT_U2;   // 16 decoded values per vector
 		if(kiter <= 1) return V_DOT_U8;
 		if(kiter <= 2) return V_DOT_U4;
 		if(kiter <= 4) return V_DOT_U2;
@@ -295,10 +297,9 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 	if(f.I > 1 && f.sAi == 1 && f.sCi == 1) {
 		if(configure(V_COL_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_COL_SPLIT;
 		configure(V_COL_U1, f, dt, a_addr, cus, flags, cfg);
-		// narrow types: a 16-byte vector already holds 8 (bf16) or 16 (fp8) columns per lane
-		if(dt.size == 2) return V_COL_U2;
-		if(dt.size == 1) return V_COL_NARROW;
 		int64_t const iv = f.I / cfg.vec;
+		// fp8: a 16-byte vector already holds 16 columns per lane
+		if(dt.size == 1) return iv >= 4 * kBlock ? V_COL_U2 : V_COL_NARROW;
 		if(iv >= 4 * kBlock) return V_COL_U4;            // wide rows: 4 vectors per lane along i
 		int64_t const per_thread = f.K / std::max(1, cfg.args.KS);
 		if(per_thread >= 8) return V_COL_U2;

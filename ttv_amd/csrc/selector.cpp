@@ -74,12 +74,16 @@ int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned f
 	float x[kMlpF], y[kMlpV];
 	features_of(f, dt, a_addr, x);
 	mlp_forward_host(w->data(), x, y);
+	// argmin of the predicted slowdown over the variants that can run the shape; the strided
+	// fallback is never timed for the training set when a dense variant exists, so it is only
+	// taken when nothing else can run
 	int best = -1;
 	Config cfg;
 	for(int v = 0; v < kMlpV; ++v) {
-		if(!configure(v, f, dt, a_addr, 256, flags, cfg)) continue;
+		if(v == V_STRIDED || !configure(v, f, dt, a_addr, 256, flags, cfg)) continue;
 		if(best < 0 || y[v] < y[best]) best = v;
 	}
+	if(best < 0 && configure(V_STRIDED, f, dt, a_addr, 256, flags, cfg)) best = V_STRIDED;
 	std::lock_guard<std::mutex> lock(g_sel_mutex);
 	if(g_cache.size() > 4096) g_cache.clear();
 	g_cache[key] = best;
