@@ -255,6 +255,7 @@ def main():
         ts = [ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(args.steps)]
         c["ms"] = float(np.mean(ts))
         c["ms_min"] = float(np.min(ts))
+        c["ms_med"] = float(np.median(ts))
         c["gbs"] = c["bytes"] / (c["ms"] * 1e-3) / 1e9
     per_case = [dict(layout=c["layout"], q=c["q"], kernel=c["kernel"], ms=round(c["ms"], 4),
                      gbs=round(c["gbs"], 1), frac=round(c["gbs"] * 1e9 / PEAK_BPS, 4)) for c in cases]
@@ -460,7 +461,7 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
                 extra = scales.numel() * 4
                 elem = 1
             ts = timed_launches(torch, ttv_amd, run, max(5, args.steps), sink, flush=flush)
-            ms = float(np.mean(ts))
+            ms = float(np.median(ts))          # speedups: medians of both (a rare stall skews a mean of ~50 us launches)
             got = Cq.as_strided([Cq.numel()], [1]).cpu().numpy().astype(np.float64)
             err = float(np.linalg.norm(got - ref) / rn)
             # error of the accumulation alone: FP64 oracle on the dequantised values
@@ -478,12 +479,13 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
                 del deq
             err_acc = float(np.linalg.norm(got - ref_q) / np.linalg.norm(ref_q))
             nbytes = case_bytes(128, elem=elem, extra=extra)
-            res[fmt].append(dict(layout=c["layout"], q=q, ms=round(ms, 4), speedup=round(c["ms"] / ms, 3),
+            res[fmt].append(dict(layout=c["layout"], q=q, ms=round(ms, 4), ms_mean=round(float(np.mean(ts)), 4),
+                                 speedup=round(c["ms_med"] / ms, 3),
                                  gbs=round(nbytes / (ms * 1e-3) / 1e9, 1),
                                  ideal_speedup=round(c["bytes"] / nbytes, 3),
                                  rel_err_vs_fp64=err, rel_err_vs_fp64_dequantised=err_acc, fp32_rel_err=err32))
             log("  %s %-5s q=%d %.4f ms  speedup %.2f (ideal %.2f)  rel err %.2e (accumulation %.2e)" % (
-                fmt, c["layout"], q, ms, c["ms"] / ms, c["bytes"] / nbytes, err, err_acc))
+                fmt, c["layout"], q, ms, c["ms_med"] / ms, c["bytes"] / nbytes, err, err_acc))
             if fmt == "fp8":
                 del codes, scales
             del Cq

@@ -197,6 +197,8 @@ struct Args {
 	int64_t fps;              // fibers per scale
 	// work assignment: 0 = grid-stride over items/groups, 1 = each block one contiguous range
 	int contig;
+	// column items with one k step per lane and no LDS reduction: double-buffered item loop
+	int one_step;
 };
 
 // Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
@@ -279,6 +281,10 @@ __device__ __forceinline__ void col_unit(const Args& P, int64_t og, int64_t itil
 		i = (itile * P.IT + it) * VEC;
 	}
 }
+template<typename Acc, int VEC, int U>
+__device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, int64_t og, int64_t itile, int os,
+                                          int it, int ks, int t, bool active, Acc (&v)[U][VEC], Acc* red);
+
 template<typename S, int VEC, int U, bool KAHAN>
 __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
                                          void* __restrict__ C, const Args& P, int64_t item, int64_t kb, int64_t ke,
@@ -356,7 +362,15 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 	for(int u = 0; u < U; ++u)
 #pragma unroll
 		for(int e = 0; e < VEC; ++e) v[u][e] = acc[u][e].value();
+	col_store<Acc, VEC, U>(C, P, og, itile, os, it, ks, t, active, v, red);
+}
 
+// Sums the lanes' partial sums of one column item over its KS lane groups and stores the item's
+// outputs (shuffles inside a wave, or LDS across waves).
+template<typename Acc, int VEC, int U>
+__device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, int64_t og, int64_t itile, int os,
+                                          int it, int ks, int t, bool active, Acc (&v)[U][VEC], Acc* red)
+{
 	if(P.red == 1) {
 		// lanes it + IT*ks of one wave hold partial sums of the same U*VEC outputs
 		Acc w[U * VEC];
@@ -408,10 +422,67 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 
 	if(active && ks == 0) {
 #pragma unroll
-		for(int u = 0; u < U; ++u)
-			if(o[u] < P.O && iu[u] < P.I)
+		for(int u = 0; u < U; ++u) {
+			int64_t oo, ii;
+			col_unit(P, og, itile, os, it, u, U, VEC, oo, ii);
+			if(oo < P.O && ii < P.I)
 #pragma unroll
-				for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, o[u], iu[u] + e, v[u][e]);
+				for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, oo, ii + e, v[u][e]);
+		}
+	}
+}
+
+// Items whose contraction is one k step per lane (K <= KS, shuffle or no reduction): the next
+// item's vectors are loaded before the current item is summed and stored, so loads stay in
+// flight through the epilogue (short contractions over narrow slabs have little else to overlap).
+template<typename S, int VEC, int U>
+__device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                    void* __restrict__ C, const Args& P, int64_t ib, int64_t ie,
+                                                    int64_t is, int64_t kb, int64_t ke)
+{
+	using Acc = typename Traits<S>::acc;
+	int const t  = static_cast<int>(threadIdx.x);
+	int const it = t % P.IT;
+	int const ks = (t / P.IT) % P.KS;
+	int const os = t / (P.IT * P.KS);
+	bool const active = os < P.OS;
+	int64_t const k = kb + ks;
+	bool const live = active && k < ke;          // lanes past the end of the contraction add zero
+	Acc const bk = live ? b[k] : Acc(0);
+	auto load = [&](int64_t item, Vec<S, VEC> (&raw)[U]) {
+		int64_t const og = item / P.iTiles, itile = item - og * P.iTiles;
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			int64_t o, i;
+			col_unit(P, og, itile, os, it, u, U, VEC, o, i);
+			int64_t const oc = o < P.O ? o : P.O - 1;
+			int64_t const ic = i < P.I ? i : P.I - VEC;
+			if(live) raw[u].load(A + oc * P.sAo + k * P.sAk + ic);
+			else
+#pragma unroll
+				for(int w = 0; w < Vec<S, VEC>::words; ++w) raw[u].w[w] = 0u;
+		}
+	};
+	auto finish = [&](int64_t item, const Vec<S, VEC> (&raw)[U]) {
+		int64_t const og = item / P.iTiles, itile = item - og * P.iTiles;
+		Acc v[U][VEC];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			Acc x[VEC];
+			decode(raw[u], x);
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) v[u][e] = x[e] * bk;
+		}
+		col_store<Acc, VEC, U>(C, P, og, itile, os, it, ks, t, active, v, nullptr);
+	};
+	Vec<S, VEC> ra[U], rb[U];
+	if(ib < ie) load(ib, ra);
+	for(int64_t item = ib; item < ie; item += 2 * is) {
+		bool const has1 = item + is < ie;
+		if(has1) load(item + is, rb);
+		finish(item, ra);
+		if(item + 2 * is < ie) load(item + 2 * is, ra);
+		if(has1) finish(item + is, rb);
 	}
 }
 
@@ -426,6 +497,10 @@ __global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const t
 	int64_t const ke = kb + P.kchunk < P.K ? kb + P.kchunk : P.K;
 	int64_t ib, ie, is;
 	block_range(P.contig, P.items, ib, ie, is);
+	if(P.one_step) {
+		col_items_pipelined<S, VEC, U>(A, b, C, P, ib, ie, is, kb, ke);
+		return;
+	}
 	for(int64_t item = ib; item < ie; item += is)
 		col_item<S, VEC, U, KAHAN>(A, b, C, P, item, kb, ke, red);
 }

@@ -12,11 +12,14 @@ namespace ttvx
 
 // Launches kernel with the configuration's grid, or, for contiguous schedules, with as many
 // blocks as the GPU holds at once (the kernel's occupancy), so every block owns one range.
+#ifndef TTVX_CONTIG_RESIDENT
+#define TTVX_CONTIG_RESIDENT 1
+#endif
 template<typename Kern, typename... Xs>
 void launch_sched(Kern kernel, const Config& cfg, hipStream_t s, Xs... xs)
 {
 	unsigned gx = static_cast<unsigned>(cfg.grid);
-	if(cfg.args.contig) {
+	if(cfg.args.contig && TTVX_CONTIG_RESIDENT) {
 		int nb = 0;
 		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, cfg.shmem) != hipSuccess || nb < 1) nb = 1;
 		int64_t const work = cfg.kind == KIND_DOT ? (cfg.args.groups + kBlock / 64 - 1) / (kBlock / 64) : cfg.args.items;

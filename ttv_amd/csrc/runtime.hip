@@ -152,7 +152,18 @@ std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 #ifndef TTVX_CONTIG_DEFAULT
 #define TTVX_CONTIG_DEFAULT 0
 #endif
-constexpr int kGridPerCU = 16;   // resident-block cap of the grid-stride loops
+#ifndef TTVX_GRID_PER_CU
+#define TTVX_GRID_PER_CU 16
+#endif
+constexpr int kGridPerCU = TTVX_GRID_PER_CU;   // block cap (per CU) of the work loops
+#ifndef TTVX_COL_WIDE_U
+#define TTVX_COL_WIDE_U 4
+#endif
+#ifndef TTVX_DOT_GROUPS_PER_WAVE
+#define TTVX_DOT_GROUPS_PER_WAVE 2
+#endif
+constexpr int kDotGroupsPerWave = TTVX_DOT_GROUPS_PER_WAVE;
+constexpr int kDotGridPerCU = 64;
 } // namespace
 
 bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus, unsigned flags, Config& cfg)
@@ -208,7 +219,12 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		P.kchunk = kchunk;
 		P.kiter  = static_cast<int>(std::min<int64_t>(1 << 30, ceil_div(std::min(kchunk, f.K), kstride)));
 		cfg.ksplit = static_cast<int>(ksplit);
-		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, kBlock / 64), gridcap)));
+		// about kDotGroupsPerWave row groups per wave: enough waves to fill the GPU, and each wave
+		// still has a next group to load while it sums the current one
+		// (4 for the narrow types, whose groups hold more rows)
+		int64_t const dot_cap = static_cast<int64_t>(std::max(cus, 1)) * kDotGridPerCU;
+		int64_t const per_wave = dt.size >= 4 ? kDotGroupsPerWave : 2 * kDotGroupsPerWave;
+		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, (kBlock / 64) * per_wave), dot_cap)));
 	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE) {
 		if(!col_shape) return false;
 		cfg.kind = KIND_COL;
@@ -224,7 +240,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		P.KS = static_cast<int>(v == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
 		static const int us[] = {1, 2, 4, 1, 1};
-		cfg.U = v == V_COL_WIDE ? 1 : us[v - V_COL_U1];
+		cfg.U = v == V_COL_WIDE ? TTVX_COL_WIDE_U : us[v - V_COL_U1];
 		// wide rows (a full-width block still leaves U vectors per lane): the U units of a lane
 		// step along i, so a block streams U*IT*16 bytes of every row it touches
 		P.along_i = (cfg.U > 1 && iv >= static_cast<int64_t>(P.IT) * cfg.U && P.IT == kBlock) ? 1 : 0;
@@ -273,6 +289,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		else P.red = 2;
 		cfg.shmem = P.red == 2 ? static_cast<std::size_t>(cfg.U) * cfg.vec * kBlock * acc_size(dt) : 0;
 		if(cfg.shmem > 64 * 1024) return false;
+		P.one_step = (P.red != 2 && P.kchunk <= P.KS && P.K <= P.KS) ? 1 : 0;
 	}
 	if(fp64_final && dt.code != TTVX_F64) P.part64 = 1;
 	return true;
@@ -283,11 +300,16 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 	Config cfg;
 	int const cus = 256;
 	if(!f.three_d) return V_STRIDED;
+	// FP64 below ~192 MiB: the launch is short and its ramp and tail weigh more, so the variants
+	// that spread the work over the most waves win (measured on 256^3: U=1 tiles 1.5x faster than
+	// U=4); for 4-byte and narrower types the same rule loses on average, and is not applied
+	bool const small = dt.size == 8 && f.O * f.K * f.I * dt.size < (int64_t{192} << 20);
 	if(f.I == 1 && (f.sAk == 1 || f.K == 1)) {
 		// rows of K elements: enough rows in flight per lane, split long rows when rows are few
 		if(configure(V_DOT_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_DOT_SPLIT;
 		configure(V_DOT_U1, f, dt, a_addr, cus, flags, cfg);
 		int64_t const kiter = cfg.args.kiter;
+		if(small && kiter >= 2) return V_DOT_U1;
 		if(dt.size == 1) return kiter <= 2 ? V_DOT_U4 : V_DOT_U2;   // 16 decoded values per vector
 		if(kiter <= 1) return V_DOT_U8;
 		if(kiter <= 2) return V_DOT_U4;
@@ -297,6 +319,7 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 	if(f.I > 1 && f.sAi == 1 && f.sCi == 1) {
 		if(configure(V_COL_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_COL_SPLIT;
 		configure(V_COL_U1, f, dt, a_addr, cus, flags, cfg);
+		if(small) return cfg.args.items >= cus ? V_COL_U1 : V_COL_NARROW;
 		int64_t const iv = f.I / cfg.vec;
 		// fp8: a 16-byte vector already holds 16 columns per lane
 		if(dt.size == 1) return iv >= 4 * kBlock ? V_COL_U2 : V_COL_NARROW;
@@ -342,15 +365,15 @@ int select_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned fl
 void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
 {
 	for(int i = 0; i < TTVX_NUM_FEATURES; ++i) out[i] = 0.f;
-	auto lg = [](int64_t x) { return x > 0 ? static_cast<float>(std::log2(static_cast<double>(x))) : 0.f; };
+	auto lg = [](double x) { return x > 0 ? static_cast<float>(std::log2(x)) : 0.f; };
 	int64_t N = 1;
 	for(auto n : f.n_layout) N *= n;
 	out[0] = static_cast<float>(f.p);
 	out[1] = f.p > 1 ? static_cast<float>(f.qpos) / static_cast<float>(f.p - 1) : 0.f;
-	out[2] = lg(f.I);
-	out[3] = lg(f.K);
-	out[4] = lg(f.O);
-	out[5] = lg(N);
+	out[2] = lg(static_cast<double>(f.I));
+	out[3] = lg(static_cast<double>(f.K));
+	out[4] = lg(static_cast<double>(f.O));
+	out[5] = lg(static_cast<double>(N));
 	out[6] = dt.code == TTVX_F32 ? 1.f : 0.f;
 	out[7] = dt.code == TTVX_BF16 ? 1.f : 0.f;
 	out[8] = dt.code == TTVX_FP8E4M3 ? 1.f : 0.f;
@@ -358,8 +381,38 @@ void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
 	out[10] = f.I == 1 ? 1.f : 0.f;
 	out[11] = static_cast<float>(pick_vec(dt, a_addr, f.I > 1 ? f.I : f.K, {})) / static_cast<float>(dt.vmax);
 	out[12] = f.three_d ? 1.f : 0.f;
-	for(int s = 0; s < 10 && s < static_cast<int>(f.n_layout.size()); ++s) out[13 + s] = lg(f.n_layout[s]);
+	out[13] = lg(static_cast<double>(N) * dt.size);
 	out[23] = static_cast<float>(f.qpos);
+	if(!f.three_d) return;
+	// tiling of the candidate kernels: parallelism, tile utilisation, work per lane
+	Config c1, c2, c4;
+	int const cus = 256;
+	if(configure(V_COL_U1, f, dt, a_addr, cus, 0u, c1)) {
+		int64_t const iv = f.I / c1.vec;
+		out[14] = lg(static_cast<double>(c1.args.items));
+		out[17] = static_cast<float>(static_cast<double>(iv) / static_cast<double>(c1.args.iTiles * c1.args.IT));
+		if(configure(V_COL_U2, f, dt, a_addr, cus, 0u, c2))
+			out[16] = static_cast<float>(static_cast<double>(f.O * iv) /
+			                             static_cast<double>(c2.args.items * c2.args.OS * c2.U * c2.args.IT));
+		if(configure(V_COL_U4, f, dt, a_addr, cus, 0u, c4))
+			out[15] = static_cast<float>(static_cast<double>(f.O * iv) /
+			                             static_cast<double>(c4.args.items * c4.args.OS * c4.U * c4.args.IT));
+		out[18] = lg(static_cast<double>(iv));
+		out[19] = lg(static_cast<double>(f.K) / c1.args.KS);
+		out[20] = static_cast<float>(c1.args.red);
+		out[21] = lg(static_cast<double>(f.O) * f.I);
+		out[22] = lg(static_cast<double>(f.K) * f.I * dt.size);
+	} else if(configure(V_DOT_U1, f, dt, a_addr, cus, 0u, c1)) {
+		int64_t const vsteps = (f.K + c1.vec - 1) / c1.vec;
+		out[14] = lg(static_cast<double>(c1.args.groups));
+		out[15] = out[16] = out[17] = static_cast<float>(static_cast<double>(vsteps) /
+		                                                 static_cast<double>(c1.args.LPR * c1.args.kiter));
+		out[18] = lg(static_cast<double>(vsteps));
+		out[19] = static_cast<float>(c1.args.kiter);
+		out[20] = lg(static_cast<double>(c1.args.LPR));
+		out[21] = lg(static_cast<double>(f.O));
+		out[22] = lg(static_cast<double>(f.K) * dt.size);
+	}
 }
 
 // ---- launch ----------------------------------------------------------------------------------

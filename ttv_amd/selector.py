@@ -27,7 +27,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(HERE, "data", "selector_gfx950.npz")
 NV = 13
 NF = 24
-DT_CODE = {"f32": _lib.F32, "bf16": _lib.BF16}
+DT_CODE = {"f32": _lib.F32, "bf16": _lib.BF16, "f64": _lib.F64, "fp8": _lib.FP8E4M3}
+DT_NAME = {v: k for k, v in DT_CODE.items()}
+ELEM = {"f32": 4, "bf16": 2, "f64": 8, "fp8": 1}
 
 
 def _ptr(t):
@@ -42,17 +44,23 @@ def _stream():
 # ---- exhaustive timing ---------------------------------------------------------------------
 
 class TimingPool:
-    def __init__(self, pool_bytes=2 << 30, b_elems=1 << 28, c_elems=1 << 27, seed=0):
+    def __init__(self, pool_bytes=2 << 30, b_elems=1 << 28, c_elems=1 << 28, seed=0):
         import torch
         dev = torch.device("cuda:0")
         g = torch.Generator(device=dev)
         g.manual_seed(seed)
         self.pool_bytes = pool_bytes
         self.A = torch.empty(pool_bytes // 4, dtype=torch.float32, device=dev).uniform_(-1, 1, generator=g)
-        # bf16 view: the upper halves of the floats are valid bfloat16 values in [-1, 1]
+        # bf16 view: the upper halves of the floats are valid bfloat16 values in [-1, 1]; the FP64
+        # and FP8 views reinterpret the same bytes (the time of a product does not depend on values)
         self.A16 = self.A.view(torch.uint16)
+        self.A64 = self.A.view(torch.float64)
+        self.A8 = self.A.view(torch.uint8)
         self.b = torch.empty(b_elems, dtype=torch.float32, device=dev).uniform_(-1, 1, generator=g)
+        self.b64 = torch.empty(b_elems // 2, dtype=torch.float64, device=dev).uniform_(-1, 1, generator=g)
         self.C = torch.empty(c_elems, dtype=torch.float32, device=dev)
+        self.C64 = self.C.view(torch.float64)
+        self.S = torch.ones(c_elems, dtype=torch.float32, device=dev)     # FP8 scales
         self.sink = torch.zeros(1, dtype=torch.int32, device=dev)
         self.offset = 0
 
@@ -75,14 +83,29 @@ def _launcher(pool, shape):
     ncount = int(np.prod(nc, dtype=np.int64))
     C = pool.C[:ncount].as_strided(nc, wc)
     b = pool.b[:na[q - 1]]
-    if shape["dtype"] == "f32":
+    dt = shape["dtype"]
+    if dt == "f32":
         def run(off):
             A = pool.A[off // 4:].as_strided(na, wa)
             ttv_amd.ttv(q, A, b, out=C)
-    else:
+    elif dt == "bf16":
         def run(off):
             A = pool.A16[off // 2:].as_strided(na, wa)
             ttv_amd.ttv_bf16(q, A, b, out=C)
+    elif dt == "f64":
+        C64 = pool.C64[:ncount].as_strided(nc, wc)
+        b64 = pool.b64[:na[q - 1]]
+
+        def run(off):
+            A = pool.A64[off // 8:].as_strided(na, wa)
+            ttv_amd.ttv(q, A, b64, out=C64)
+    else:
+        fps = ttv_amd.default_fibers_per_scale(na[q - 1])
+        S = pool.S[:int(lib().ttvx_fp8_num_scales(q, len(na), _lib.sizes(na), _lib.sizes(wa), _lib.sizes(pia), fps))]
+
+        def run(off):
+            A = pool.A8[off:].as_strided(na, wa)
+            ttv_amd.ttv_fp8(q, A, S, fps, b, out=C)
     return run
 
 
@@ -101,7 +124,7 @@ def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0):
     """Median time (ms) of every valid variant (NaN for invalid ones)."""
     import torch
     import ttv_amd
-    elem = 4 if shape["dtype"] == "f32" else 2
+    elem = ELEM[shape["dtype"]]
     nbytes = int(np.prod(shape["na"], dtype=np.int64)) * elem
     run = _launcher(pool, shape)
     vs = valid_variants(shape)
@@ -171,8 +194,7 @@ def shapes_from_arrays(z):
         p, q, code = (int(x) for x in z["meta"][k])
         na = [int(x) for x in z["na"][k, :p]]
         pia = [int(x) for x in z["pia"][k, :p]]
-        dt = "f32" if code == _lib.F32 else "bf16"
-        out.append(dict(na=na, q=q, pia=pia, wa=generate_strides(na, pia), dtype=dt))
+        out.append(dict(na=na, q=q, pia=pia, wa=generate_strides(na, pia), dtype=DT_NAME[code]))
     return out
 
 
@@ -289,7 +311,9 @@ def load_weights(path=WEIGHTS_PATH):
 
 def main():
     """Collects timings of train/held-out shapes, trains on the GPU, reports regrets, saves the
-    weights and the timing tables:  python -m ttv_amd.selector --out DIR [--train N --test M]."""
+    weights and the timing tables:
+        python -m ttv_amd.selector --out DIR [--train N --test M]
+        python -m ttv_amd.selector --out DIR --tables DIR0     (retrain on stored tables)"""
     import argparse
     import json
     ap = argparse.ArgumentParser()
@@ -298,35 +322,53 @@ def main():
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default="gpurun_out/selector")
+    ap.add_argument("--tables", help="directory with timings_train.npz / timings_test.npz to train on")
     a = ap.parse_args()
     import torch
-    from .workloads import selector_shapes
+    from .workloads import selector_shapes, structured_shapes
 
     def log(*x):
         print(*x, flush=True)
 
     os.makedirs(a.out, exist_ok=True)
-    shapes_tr = selector_shapes(seed=101, count=a.train)
-    shapes_te = selector_shapes(seed=202, count=a.test)
-    pool = TimingPool()
     t0 = time.time()
-    f_te, t_te, h_te = collect_timings(shapes_te, reps=a.reps, log=log, pool=pool)
-    log("held-out timings done in %.0f s" % (time.time() - t0))
-    np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te,
-                        **shape_arrays(shapes_te))
-    f_tr, t_tr, h_tr = collect_timings(shapes_tr, reps=a.reps, log=log, pool=pool)
-    log("training timings done in %.0f s" % (time.time() - t0))
-    np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
-                        **shape_arrays(shapes_tr))
-    del pool
-    torch.cuda.empty_cache()
+    if a.tables:
+        ztr = np.load(os.path.join(a.tables, "timings_train.npz"))
+        zte = np.load(os.path.join(a.tables, "timings_test.npz"))
+        f_tr, t_tr, h_tr, n_struct = ztr["feats"], ztr["times"], ztr["heur"], int(ztr["n_structured"])
+        f_te, t_te, h_te = zte["feats"], zte["times"], zte["heur"]
+        n_tr = len(f_tr) - n_struct
+    else:
+        shapes_te = selector_shapes(seed=202, count=a.test)
+        shapes_st = structured_shapes()
+        shapes_tr = selector_shapes(seed=101, count=a.train) + shapes_st
+        n_tr, n_struct = a.train, len(shapes_st)
+        pool = TimingPool()
+        f_te, t_te, h_te = collect_timings(shapes_te, reps=a.reps, log=log, pool=pool)
+        log("held-out timings done in %.0f s" % (time.time() - t0))
+        np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te,
+                            **shape_arrays(shapes_te))
+        f_tr, t_tr, h_tr = collect_timings(shapes_tr, reps=a.reps, log=log, pool=pool)
+        log("training timings done in %.0f s" % (time.time() - t0))
+        np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
+                            n_structured=n_struct, **shape_arrays(shapes_tr))
+        del pool
+        torch.cuda.empty_cache()
     params = train(f_tr, t_tr, epochs=a.epochs, log=log)
     ch_te, _ = predict_gpu(params, f_te, t_te)
     ch_tr, _ = predict_gpu(params, f_tr, t_tr)
+    st = slice(n_tr, len(f_tr))
     rep = dict(regret_learned_test=regret(ch_te, t_te), regret_heuristic_test=regret(h_te, t_te),
                regret_learned_train=regret(ch_tr, t_tr), regret_heuristic_train=regret(h_tr, t_tr),
-               n_train=len(shapes_tr), n_test=len(shapes_te), seconds=time.time() - t0,
-               device=torch.cuda.get_device_properties(0).gcnArchName, hip=torch.version.hip)
+               regret_learned_structured=regret(ch_tr[st], t_tr[st]),
+               regret_heuristic_structured=regret(h_tr[st], t_tr[st]),
+               n_train=int(len(f_tr)), n_structured=int(n_struct), n_test=int(len(f_te)),
+               seconds=time.time() - t0, device=torch.cuda.get_device_properties(0).gcnArchName, hip=torch.version.hip)
+    for name, code in DT_CODE.items():
+        sel = f_te[:, 6 + [_lib.F32, _lib.BF16, _lib.FP8E4M3, _lib.F64].index(code)] > 0.5
+        if sel.any():
+            rep["regret_learned_test_" + name] = regret(ch_te[sel], t_te[sel])
+            rep["regret_heuristic_test_" + name] = regret(h_te[sel], t_te[sel])
     log(json.dumps(rep))
     save(os.path.join(a.out, "selector_gfx950.npz"), params, dict(report=json.dumps(rep)))
     with open(os.path.join(a.out, "report.json"), "w") as f:

@@ -28,7 +28,24 @@ def batch_problems(seed, count=512, cap=1 << 26, lo=16, hi=512):
     return out
 
 
-def selector_shapes(seed, count, lo_log2=20, hi_log2=28, pmin=2, pmax=10, dtypes=("f32", "bf16")):
+def structured_shapes(dtypes=("f32", "bf16", "f64", "fp8"), sizes_log2=(20, 22, 24, 26, 28), pmax=10):
+    """Equal extents n^p (the shapes of the reference's benchmarks, e.g. 128^4 and 16^7) for
+    p = 2..pmax and N = 2^L, every q, first- and last-order layouts, every storage type."""
+    out = []
+    for p in range(2, pmax + 1):
+        for L in sizes_log2:
+            n = max(2, int(round(2.0 ** (L / p))))
+            while n ** p > (1 << max(sizes_log2)):
+                n -= 1
+            na = [n] * p
+            for pia in (list(range(1, p + 1)), list(range(p, 0, -1))):
+                for q in range(1, p + 1):
+                    for dt in dtypes:
+                        out.append(dict(na=na, q=q, pia=pia, wa=generate_strides(na, pia), dtype=dt))
+    return out
+
+
+def selector_shapes(seed, count, lo_log2=20, hi_log2=28, pmin=2, pmax=10, dtypes=("f32", "bf16", "f64", "fp8")):
     rng = np.random.default_rng(seed)
     out = []
     while len(out) < count:
