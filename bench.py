@@ -317,6 +317,7 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(np, a_host, cases, args, log)
+        out["host_staged"] = host_staged_leg(np, ttv_amd, a_host, cases, log)
 
     if not args.no_batch:
         del a_host
@@ -326,6 +327,24 @@ def main():
         out["other_configs"] = other_configs_leg(torch, np, ttv_amd, Aflat, log)
 
     print(json.dumps(out), flush=True)
+
+
+def host_staged_leg(np, ttv_amd, a_host, cases, log, reps=3):
+    """The drop-in synchronous entry point with host buffers (what an unchanged reference caller
+    does): A (1 GiB) is staged over PCIe on every call, so this rate is bounded by the link."""
+    c = cases[0]
+    A = np.lib.stride_tricks.as_strided(a_host, shape=SHAPE, strides=[w * 4 for w in c["wa"]])
+    b = c["b"].cpu().numpy()
+    ttv_amd.ttv(c["q"], A, b)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ttv_amd.ttv(c["q"], A, b)
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.median(ts)) * 1e3
+    gbs = c["bytes"] / (ms * 1e-3) / 1e9
+    log("host-staged drop-in call (%s q=%d): %.2f ms, %.1f GB/s including PCIe" % (c["layout"], c["q"], ms, gbs))
+    return dict(layout=c["layout"], q=c["q"], ms=round(ms, 3), gbs_including_pcie=round(gbs, 1))
 
 
 def other_configs_leg(torch, np, ttv_amd, Aflat, log, reps=5):
