@@ -149,6 +149,10 @@ int ttvx_ttv_fp8_async(size_t q, size_t p,
 #define TTVX_SELECT_HEURISTIC 0  /* fixed rule                                     */
 #define TTVX_SELECT_LEARNED   1  /* MLP selector (falls back to the rule without weights) */
 #define TTVX_SELECT_FORCED    2  /* the variant given to ttvx_set_selection      */
+#define TTVX_SELECT_AUTO      3  /* default: the MLP selector with the weights set through
+                                    ttvx_selector_set_weights, else the shipped weights (the file
+                                    named by $TTVX_SELECTOR_WEIGHTS, else data/selector_gfx950.bin
+                                    next to the library's lib/ directory), else the fixed rule */
 
 #define TTVX_NUM_FEATURES 24
 

@@ -57,7 +57,7 @@ def test_batch_matches_single_launches_and_oracle(ttvx, device):
                 err = np.linalg.norm(batched[n] - ref) / np.linalg.norm(ref)
                 assert err <= 1e-5, (n, specs[n], err)
     finally:
-        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
+        ttvx.restore_default_selection()
     bt.close()
 
 

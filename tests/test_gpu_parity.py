@@ -67,14 +67,14 @@ def test_known_answer_grid_every_variant(ttvx, device, dt):
                         variants = [-1] + [v for v in range(13) if ttvx.variant_valid(q, na, wa, pia, v, code)]
                         for v in variants:
                             if v < 0:
-                                ttvx.set_selection(ttvx.SELECT_HEURISTIC)
+                                ttvx.set_selection(ttvx.SELECT_AUTO)
                             else:
                                 ttvx.set_selection(ttvx.SELECT_FORCED, v)
                             C = ttvx.ttv(q, A, b)
                             got = _flat(C)
                             assert np.array_equal(got.astype(np.float64), exp), (na, pia, q, v)
     finally:
-        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
+        ttvx.restore_default_selection()
 
 
 def test_reference_fixtures(ttvx, device):
@@ -130,7 +130,7 @@ def test_random_shapes_every_variant(ttvx, device):
                     C = ttvx.ttv(q, A, torch.from_numpy(b).to(device))
                     assert _rel(_flat(C), ref) <= TOL[dt], (na, pia, q, v, dt)
     finally:
-        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
+        ttvx.restore_default_selection()
 
 
 def test_unaligned_and_strided_views(ttvx, device):

@@ -69,7 +69,6 @@ def test_gpu_and_host_inference_agree(ttvx, device):
     try:
         host = [ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"])["variant"] for s in shapes]
     finally:
-        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
-        ttvx.set_selector_weights(None)
+        ttvx.restore_default_selection()
     agree = np.mean(np.array(host) == choice)
     assert agree >= 0.99, agree

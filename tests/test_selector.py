@@ -39,11 +39,11 @@ def test_learned_selector_beats_fixed_rule_on_held_out_shapes(ttvx):
     try:
         learned = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"]
                             for s in shapes])
-    finally:
         ttvx.set_selection(ttvx.SELECT_HEURISTIC)
-        ttvx.set_selector_weights(None)
-    fixed = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"]
-                      for s in shapes])
+        fixed = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"]
+                          for s in shapes])
+    finally:
+        ttvx.restore_default_selection()
     assert np.all(np.isfinite(times[np.arange(len(learned)), learned])), "learned pick must be a timed variant"
     r_learned, r_fixed = _regret(learned, times), _regret(fixed, times)
     print("held-out regret: learned %.4f, fixed rule %.4f" % (r_learned, r_fixed))
@@ -67,7 +67,33 @@ def test_host_inference_matches_fp64_restatement(ttvx):
                          for s in shapes])
         feats_now = np.stack([ttvx.features(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]]) for s in shapes])
     finally:
-        ttvx.set_selection(ttvx.SELECT_HEURISTIC)
-        ttvx.set_selector_weights(None)
+        ttvx.restore_default_selection()
     assert np.array_equal(feats_now, feats)
     assert np.mean(host == pick64) >= 0.99
+
+
+def test_library_loads_the_shipped_weights_by_default():
+    """A fresh process selects with the MLP and the shipped weights (data/selector_gfx950.bin,
+    found next to the library) without any call: same picks as with the weights set explicitly."""
+    import subprocess
+    import sys
+    code = r'''
+import sys, numpy as np
+sys.path.insert(0, %r)
+import ttv_amd
+from ttv_amd import selector as S
+z = np.load(%r)
+shapes = S.shapes_from_arrays(z)[:300]
+pick = lambda: [ttv_amd.plan_info(s["q"], s["na"], s["wa"], s["pia"], S.DT_CODE[s["dtype"]])["variant"] for s in shapes]
+default = pick()
+ttv_amd.set_selector_weights(S.load_weights()); ttv_amd.set_selection(ttv_amd.SELECT_LEARNED)
+learned = pick()
+ttv_amd.set_selection(ttv_amd.SELECT_HEURISTIC)
+fixed = pick()
+print(int(default == learned), int(default != fixed))
+''' % (REPO, TEST_TABLE)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    same, differs = out.stdout.split()[-2:]
+    assert same == "1", "default picks differ from the shipped weights' picks"
+    assert differs == "1", "default picks equal the fixed rule's: the weights were not loaded"

@@ -19,15 +19,16 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (BF16, F32, F64, FLAG_FP64_FINAL, FLAG_KAHAN, FP8E4M3, SELECT_FORCED, SELECT_HEURISTIC,
-                   SELECT_LEARNED, TTVError, check, lib, sizes)
+from ._lib import (BF16, F32, F64, FLAG_FP64_FINAL, FLAG_KAHAN, FP8E4M3, SELECT_AUTO, SELECT_FORCED,
+                   SELECT_HEURISTIC, SELECT_LEARNED, TTVError, check, lib, sizes)
 from .meta import (generate_k_order_layout, generate_output_layout, generate_output_shape, generate_strides,
                    layout_from_strides)
 
 __all__ = [
     "ttv", "ttvs", "quantize_bf16", "quantize_fp8", "ttv_bf16", "ttv_fp8", "plan_info", "variant_names",
     "set_selection", "variant_valid", "features", "TTVError", "F32", "F64", "BF16", "FP8E4M3", "FLAG_KAHAN",
-    "FLAG_FP64_FINAL", "SELECT_HEURISTIC", "SELECT_LEARNED", "SELECT_FORCED", "generate_output_shape",
+    "FLAG_FP64_FINAL", "SELECT_HEURISTIC", "SELECT_LEARNED", "SELECT_FORCED", "SELECT_AUTO",
+    "restore_default_selection", "generate_output_shape",
     "generate_output_layout", "generate_strides", "generate_k_order_layout", "layout_from_strides",
 ]
 
@@ -259,6 +260,15 @@ def features(q, na, wa, pia, dtype=F32):
     out = (ctypes.c_float * _lib.NUM_FEATURES)()
     check(lib().ttvx_features(dtype, q, len(na), sizes(na), sizes(wa), sizes(pia), out))
     return np.array(out[:], dtype=np.float32)
+
+
+def restore_default_selection():
+    """Back to the library's default: the MLP selector with the shipped weights (the fixed rule
+    when the weights file is absent)."""
+    from .selector import load_weights
+    params = load_weights()
+    set_selector_weights(params)
+    set_selection(SELECT_AUTO)
 
 
 def set_selector_weights(params):

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libttvx.so")
 # element types / flags / selection modes (ttvx.h)
 F32, F64, BF16, FP8E4M3 = 0, 1, 2, 3
 FLAG_KAHAN, FLAG_FP64_FINAL = 1, 2
-SELECT_HEURISTIC, SELECT_LEARNED, SELECT_FORCED = 0, 1, 2
+SELECT_HEURISTIC, SELECT_LEARNED, SELECT_FORCED, SELECT_AUTO = 0, 1, 2, 3
 NUM_FEATURES = 24
 
 # error codes

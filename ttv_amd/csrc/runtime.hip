@@ -333,13 +333,14 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 
 namespace
 {
-std::atomic<int> g_sel_mode{TTVX_SELECT_HEURISTIC};
+std::atomic<int> g_sel_mode{TTVX_SELECT_AUTO};
 std::atomic<int> g_forced{0};
 } // namespace
 
 int set_selection(int mode, int forced)
 {
-	if(mode != TTVX_SELECT_HEURISTIC && mode != TTVX_SELECT_LEARNED && mode != TTVX_SELECT_FORCED)
+	if(mode != TTVX_SELECT_HEURISTIC && mode != TTVX_SELECT_LEARNED && mode != TTVX_SELECT_FORCED &&
+	   mode != TTVX_SELECT_AUTO)
 		return fail(TTVX_ERR_ARGUMENT, "ttvx: unknown selection mode");
 	if(mode == TTVX_SELECT_FORCED && (forced < 0 || forced >= V_COUNT))
 		return fail(TTVX_ERR_ARGUMENT, "ttvx: forced variant out of range");
@@ -355,7 +356,8 @@ int select_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned fl
 		Config cfg;
 		int const v = g_forced.load();
 		if(configure(v, f, dt, a_addr, 256, flags, cfg)) return v;
-	} else if(mode == TTVX_SELECT_LEARNED) {
+	} else if(mode == TTVX_SELECT_LEARNED || mode == TTVX_SELECT_AUTO) {
+		if(mode == TTVX_SELECT_AUTO) load_default_selector_weights();
 		int const v = learned_variant(f, dt, a_addr, flags);
 		if(v >= 0) return v;
 	}

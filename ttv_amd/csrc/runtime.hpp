@@ -85,6 +85,9 @@ int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned f
 
 // Loads/clears the learned selector's weights (layout in mlp.hpp); count 0 clears.
 int set_selector_weights(const float* params, std::size_t count);
+// Loads the shipped selector weights once per process unless weights were set (or cleared)
+// explicitly; failure leaves the fixed rule in charge.
+void load_default_selector_weights();
 
 // Typed launchers, one per storage type (launch_<type>.hip).
 int launch_f32 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);

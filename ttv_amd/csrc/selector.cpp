@@ -1,7 +1,13 @@
 // Learned selection at plan time: host inference of the selector MLP (mlp.hpp) with the
 // weights set through ttvx_selector_set_weights, plus a per-shape cache of decisions.
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -19,6 +25,7 @@ namespace
 {
 std::mutex g_sel_mutex;
 std::shared_ptr<const std::vector<float>> g_weights;
+bool g_weights_decided = false;     // set, cleared or default load attempted
 // decision cache: (dtype, O, K, I, alignment class, qpos, p) -> variant
 using Key = std::tuple<int, int64_t, int64_t, int64_t, int, int, int>;
 std::map<Key, int> g_cache;
@@ -49,6 +56,7 @@ int set_selector_weights(const float* params, std::size_t count)
 {
 	std::lock_guard<std::mutex> lock(g_sel_mutex);
 	g_cache.clear();
+	g_weights_decided = true;
 	if(count == 0) {
 		g_weights.reset();
 		return TTVX_OK;
@@ -57,6 +65,48 @@ int set_selector_weights(const float* params, std::size_t count)
 		return fail(TTVX_ERR_ARGUMENT, "ttvx: selector weights must hold " + std::to_string(kMlpParams) + " floats");
 	g_weights = std::make_shared<const std::vector<float>>(params, params + count);
 	return TTVX_OK;
+}
+
+namespace
+{
+// raw little-endian float32 file of exactly kMlpParams values
+bool read_weights(const std::string& path, std::vector<float>& w)
+{
+	std::FILE* fp = std::fopen(path.c_str(), "rb");
+	if(fp == nullptr) return false;
+	w.assign(kMlpParams + 1, 0.f);
+	std::size_t const n = std::fread(w.data(), sizeof(float), w.size(), fp);
+	std::fclose(fp);
+	if(n != kMlpParams) return false;
+	w.resize(kMlpParams);
+	for(float x : w)
+		if(!std::isfinite(x)) return false;
+	return true;
+}
+} // namespace
+
+void load_default_selector_weights()
+{
+	std::lock_guard<std::mutex> lock(g_sel_mutex);
+	if(g_weights_decided) return;
+	g_weights_decided = true;
+	std::vector<std::string> paths;
+	if(const char* env = std::getenv("TTVX_SELECTOR_WEIGHTS")) paths.emplace_back(env);
+	Dl_info info;
+	if(dladdr(reinterpret_cast<void*>(&load_default_selector_weights), &info) != 0 && info.dli_fname != nullptr) {
+		std::string lib = info.dli_fname;
+		std::size_t const slash = lib.find_last_of('/');
+		std::string const dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+		paths.push_back(dir + "/../data/selector_gfx950.bin");
+	}
+	for(auto const& p : paths) {
+		std::vector<float> w;
+		if(read_weights(p, w)) {
+			g_weights = std::make_shared<const std::vector<float>>(std::move(w));
+			g_cache.clear();
+			return;
+		}
+	}
 }
 
 int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags)

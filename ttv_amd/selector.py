@@ -25,6 +25,8 @@ from .meta import generate_output_layout, generate_output_shape, generate_stride
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(HERE, "data", "selector_gfx950.npz")
+WEIGHTS_BIN = os.path.join(HERE, "data", "selector_gfx950.bin")     # raw float32, loaded by libttvx itself
+STRUCT_WEIGHT = 8     # the equal-extent benchmark shapes count this many times in the training loss
 NV = 13
 NF = 24
 DT_CODE = {"f32": _lib.F32, "bf16": _lib.BF16, "f64": _lib.F64, "fp8": _lib.FP8E4M3}
@@ -254,9 +256,14 @@ class GPUTrainer:
                                   0.999, 1e-8, wd, _stream()))
 
 
-def train(feats, times, epochs=300, batch=512, lr=2e-3, seed=0, log=None):
+def train(feats, times, epochs=300, batch=512, lr=2e-3, seed=0, log=None, n_structured=0, struct_weight=1):
+    """Adam on the GPU kernels; the last n_structured rows (equal-extent shapes) are repeated
+    struct_weight times, so the common benchmark shapes weigh more than one random shape each."""
     import torch
     dev = torch.device("cuda:0")
+    if n_structured and struct_weight > 1:
+        feats = np.concatenate([feats] + [feats[-n_structured:]] * (struct_weight - 1))
+        times = np.concatenate([times] + [times[-n_structured:]] * (struct_weight - 1))
     T, M = targets(times)
     params = init_params(feats, seed)
     tr = GPUTrainer(params, batch)
@@ -298,8 +305,10 @@ def regret(choice, times):
 
 
 def save(path, params, meta):
+    """Weights as .npz (with the report) and as the raw float32 .bin the library loads by itself."""
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.savez_compressed(path, params=params.astype(np.float32), **{k: np.asarray(v) for k, v in meta.items()})
+    params.astype("<f4").tofile(os.path.splitext(path)[0] + ".bin")
 
 
 def load_weights(path=WEIGHTS_PATH):
@@ -333,8 +342,12 @@ def main():
     os.makedirs(a.out, exist_ok=True)
     t0 = time.time()
     if a.tables:
-        ztr = np.load(os.path.join(a.tables, "timings_train.npz"))
-        zte = np.load(os.path.join(a.tables, "timings_test.npz"))
+        def table(kind):
+            for name in ("timings_%s.npz" % kind, "selector_timings_%s_gfx950.npz" % kind):
+                if os.path.exists(os.path.join(a.tables, name)):
+                    return np.load(os.path.join(a.tables, name))
+            raise FileNotFoundError("no %s timing table under %s" % (kind, a.tables))
+        ztr, zte = table("train"), table("test")
         f_tr, t_tr, h_tr, n_struct = ztr["feats"], ztr["times"], ztr["heur"], int(ztr["n_structured"])
         f_te, t_te, h_te = zte["feats"], zte["times"], zte["heur"]
         n_tr = len(f_tr) - n_struct
@@ -354,7 +367,7 @@ def main():
                             n_structured=n_struct, **shape_arrays(shapes_tr))
         del pool
         torch.cuda.empty_cache()
-    params = train(f_tr, t_tr, epochs=a.epochs, log=log)
+    params = train(f_tr, t_tr, epochs=a.epochs, log=log, n_structured=n_struct, struct_weight=STRUCT_WEIGHT)
     ch_te, _ = predict_gpu(params, f_te, t_te)
     ch_tr, _ = predict_gpu(params, f_tr, t_tr)
     st = slice(n_tr, len(f_tr))
