@@ -149,10 +149,11 @@ int ttvx_ttv_fp8_async(size_t q, size_t p,
 #define TTVX_SELECT_HEURISTIC 0  /* fixed rule                                     */
 #define TTVX_SELECT_LEARNED   1  /* MLP selector (falls back to the rule without weights) */
 #define TTVX_SELECT_FORCED    2  /* the variant given to ttvx_set_selection      */
-#define TTVX_SELECT_AUTO      3  /* default: the MLP selector with the weights set through
-                                    ttvx_selector_set_weights, else the shipped weights (the file
-                                    named by $TTVX_SELECTOR_WEIGHTS, else data/selector_gfx950.bin
-                                    next to the library's lib/ directory), else the fixed rule */
+#define TTVX_SELECT_AUTO      3  /* default: the tuning table, then the MLP selector, then the fixed
+                                    rule.  Unless set explicitly, the shipped table and weights are
+                                    loaded on first use ($TTVX_SELECTOR_TABLE / $TTVX_SELECTOR_WEIGHTS,
+                                    else data/selector_table_gfx950.bin and data/selector_gfx950.bin
+                                    next to the library's lib/ directory) */
 
 #define TTVX_NUM_FEATURES 24
 
@@ -182,6 +183,11 @@ int ttvx_features(int dtype, size_t q, size_t p, const size_t* na, const size_t*
  * the network predicts log2(t_variant / t_fastest) for every variant. */
 size_t ttvx_selector_num_params(void);
 int ttvx_selector_set_weights(const float* params, size_t count);
+
+/* Tuning table of the exhaustive autotuner: `count` rows of 5 int64 (dtype, O, K, I, variant), the
+ * measured fastest variant of folded problems (O, K, I as in ttvx_plan_info); the learned and auto
+ * modes take a row's variant for a 16-byte aligned A before asking the MLP.  count 0 clears it. */
+int ttvx_selector_set_table(const int64_t* rows, size_t count);
 
 /* ---- selector network on the GPU (no counterpart in the reference) --------------------- */
 /* Device-pointer kernels that train and evaluate the selector MLP (parameter layout above).

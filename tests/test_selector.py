@@ -74,7 +74,8 @@ def test_host_inference_matches_fp64_restatement(ttvx):
 
 def test_library_loads_the_shipped_weights_by_default():
     """A fresh process selects with the MLP and the shipped weights (data/selector_gfx950.bin,
-    found next to the library) without any call: same picks as with the weights set explicitly."""
+    found next to the library) without any call: same picks as with the weights set explicitly
+    (the held-out shapes fold to no row of the tuning table)."""
     import subprocess
     import sys
     code = r'''
@@ -97,3 +98,30 @@ print(int(default == learned), int(default != fixed))
     same, differs = out.stdout.split()[-2:]
     assert same == "1", "default picks differ from the shipped weights' picks"
     assert differs == "1", "default picks equal the fixed rule's: the weights were not loaded"
+
+
+def test_tuning_table_rows_are_the_measured_fastest():
+    """The shipped tuning table holds, for the training shapes, the fastest variant the exhaustive
+    autotuner measured (pooled over shapes that fold to the same problem), and the default
+    selection returns it for those shapes."""
+    import ttv_amd
+    from ttv_amd import selector as S
+    rows = S.load_table()
+    assert rows is not None and rows.shape[1] == 5
+    table = {tuple(int(x) for x in r[:4]): int(r[4]) for r in rows}
+    z = np.load(os.path.join(DATA, "selector_timings_train_gfx950.npz"))
+    shapes = S.shapes_from_arrays(z)
+    times = z["times"]
+    pick = np.random.default_rng(0).choice(len(shapes), 200, replace=False)
+    ttv_amd.restore_default_selection()
+    slow = []
+    for n in pick:
+        s = shapes[n]
+        code = S.DT_CODE[s["dtype"]]
+        info = ttv_amd.plan_info(s["q"], s["na"], s["wa"], s["pia"], code)
+        key = (code, info["O"], info["K"], info["I"])
+        assert key in table, key
+        assert info["variant"] == table[key]
+        t = times[n]
+        slow.append(t[table[key]] / np.nanmin(t))
+    assert np.mean(slow) <= 1.01 and max(slow) <= 1.25

@@ -28,7 +28,7 @@ __all__ = [
     "ttv", "ttvs", "quantize_bf16", "quantize_fp8", "ttv_bf16", "ttv_fp8", "plan_info", "variant_names",
     "set_selection", "variant_valid", "features", "TTVError", "F32", "F64", "BF16", "FP8E4M3", "FLAG_KAHAN",
     "FLAG_FP64_FINAL", "SELECT_HEURISTIC", "SELECT_LEARNED", "SELECT_FORCED", "SELECT_AUTO",
-    "restore_default_selection", "generate_output_shape",
+    "restore_default_selection", "set_selector_table", "generate_output_shape",
     "generate_output_layout", "generate_strides", "generate_k_order_layout", "layout_from_strides",
 ]
 
@@ -263,12 +263,22 @@ def features(q, na, wa, pia, dtype=F32):
 
 
 def restore_default_selection():
-    """Back to the library's default: the MLP selector with the shipped weights (the fixed rule
-    when the weights file is absent)."""
-    from .selector import load_weights
-    params = load_weights()
-    set_selector_weights(params)
+    """Back to the library's default: the shipped tuning table, then the MLP selector with the
+    shipped weights, then the fixed rule (for what is absent)."""
+    from .selector import load_table, load_weights
+    set_selector_weights(load_weights())
+    set_selector_table(load_table())
     set_selection(SELECT_AUTO)
+
+
+def set_selector_table(rows):
+    """Tuning table rows (n x 5 int64: dtype, O, K, I, variant); None clears it."""
+    L = lib()
+    if rows is None:
+        check(L.ttvx_selector_set_table(None, 0))
+        return
+    arr = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1, 5)
+    check(L.ttvx_selector_set_table(arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), arr.shape[0]))
 
 
 def set_selector_weights(params):

@@ -49,6 +49,7 @@ _SIGS = {
     "ttvx_features": ([_int, _sz, _sz, _szp, _szp, _szp, ctypes.POINTER(ctypes.c_float)], _int),
     "ttvx_selector_num_params": ([], _sz),
     "ttvx_selector_set_weights": ([ctypes.POINTER(ctypes.c_float), _sz], _int),
+    "ttvx_selector_set_table": ([ctypes.POINTER(ctypes.c_int64), _sz], _int),
     "ttvx_bench_read": ([_vp, _sz, _vp, _sz, _vp], _int),
     "ttvx_bench_copy": ([_vp, _vp, _sz, _vp], _int),
     "ttvx_bench_spin": ([_int, _vp, _vp], _int),

@@ -88,6 +88,7 @@ int set_selector_weights(const float* params, std::size_t count);
 // Loads the shipped selector weights once per process unless weights were set (or cleared)
 // explicitly; failure leaves the fixed rule in charge.
 void load_default_selector_weights();
+int set_selector_table(const int64_t* rows, std::size_t count);
 
 // Typed launchers, one per storage type (launch_<type>.hip).
 int launch_f32 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);

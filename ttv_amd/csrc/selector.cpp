@@ -26,6 +26,10 @@ namespace
 std::mutex g_sel_mutex;
 std::shared_ptr<const std::vector<float>> g_weights;
 bool g_weights_decided = false;     // set, cleared or default load attempted
+// tuning table: measured fastest variant of folded problems (dtype, O, K, I), 16-byte aligned A
+using TKey = std::tuple<int, int64_t, int64_t, int64_t>;
+std::map<TKey, int> g_table;
+bool g_table_decided = false;
 // decision cache: (dtype, O, K, I, alignment class, qpos, p) -> variant
 using Key = std::tuple<int, int64_t, int64_t, int64_t, int, int, int>;
 std::map<Key, int> g_cache;
@@ -85,28 +89,86 @@ bool read_weights(const std::string& path, std::vector<float>& w)
 }
 } // namespace
 
+// the data/ directory beside the library's lib/ directory
+std::string shipped_data_dir()
+{
+	Dl_info info;
+	if(dladdr(reinterpret_cast<void*>(&load_default_selector_weights), &info) == 0 || info.dli_fname == nullptr)
+		return std::string();
+	std::string lib = info.dli_fname;
+	std::size_t const slash = lib.find_last_of('/');
+	std::string const dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+	return dir + "/../data/";
+}
+
+// raw little-endian int64 rows (dtype, O, K, I, variant)
+bool read_table(const std::string& path, std::map<TKey, int>& table)
+{
+	std::FILE* fp = std::fopen(path.c_str(), "rb");
+	if(fp == nullptr) return false;
+	std::vector<int64_t> rows;
+	int64_t buf[5];
+	while(std::fread(buf, sizeof(int64_t), 5, fp) == 5) rows.insert(rows.end(), buf, buf + 5);
+	std::fclose(fp);
+	table.clear();
+	for(std::size_t r = 0; r + 5 <= rows.size(); r += 5) {
+		if(rows[r + 4] < 0 || rows[r + 4] >= V_COUNT) return false;
+		table[TKey{static_cast<int>(rows[r]), rows[r + 1], rows[r + 2], rows[r + 3]}] = static_cast<int>(rows[r + 4]);
+	}
+	return !table.empty();
+}
+
 void load_default_selector_weights()
 {
 	std::lock_guard<std::mutex> lock(g_sel_mutex);
-	if(g_weights_decided) return;
-	g_weights_decided = true;
-	std::vector<std::string> paths;
-	if(const char* env = std::getenv("TTVX_SELECTOR_WEIGHTS")) paths.emplace_back(env);
-	Dl_info info;
-	if(dladdr(reinterpret_cast<void*>(&load_default_selector_weights), &info) != 0 && info.dli_fname != nullptr) {
-		std::string lib = info.dli_fname;
-		std::size_t const slash = lib.find_last_of('/');
-		std::string const dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
-		paths.push_back(dir + "/../data/selector_gfx950.bin");
-	}
-	for(auto const& p : paths) {
-		std::vector<float> w;
-		if(read_weights(p, w)) {
-			g_weights = std::make_shared<const std::vector<float>>(std::move(w));
-			g_cache.clear();
-			return;
+	std::string const data = shipped_data_dir();
+	if(!g_weights_decided) {
+		g_weights_decided = true;
+		std::vector<std::string> paths;
+		if(const char* env = std::getenv("TTVX_SELECTOR_WEIGHTS")) paths.emplace_back(env);
+		if(!data.empty()) paths.push_back(data + "selector_gfx950.bin");
+		for(auto const& p : paths) {
+			std::vector<float> w;
+			if(read_weights(p, w)) {
+				g_weights = std::make_shared<const std::vector<float>>(std::move(w));
+				g_cache.clear();
+				break;
+			}
 		}
 	}
+	if(!g_table_decided) {
+		g_table_decided = true;
+		std::vector<std::string> paths;
+		if(const char* env = std::getenv("TTVX_SELECTOR_TABLE")) paths.emplace_back(env);
+		if(!data.empty()) paths.push_back(data + "selector_table_gfx950.bin");
+		for(auto const& p : paths) {
+			std::map<TKey, int> t;
+			if(read_table(p, t)) {
+				g_table = std::move(t);
+				g_cache.clear();
+				break;
+			}
+		}
+	}
+}
+
+int set_selector_table(const int64_t* rows, std::size_t count)
+{
+	std::lock_guard<std::mutex> lock(g_sel_mutex);
+	g_cache.clear();
+	g_table_decided = true;
+	g_table.clear();
+	if(count == 0) return TTVX_OK;
+	if(rows == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx: null selector table");
+	for(std::size_t r = 0; r < count; ++r) {
+		int64_t const* x = rows + 5 * r;
+		if(x[4] < 0 || x[4] >= V_COUNT) {
+			g_table.clear();
+			return fail(TTVX_ERR_ARGUMENT, "ttvx: selector table row " + std::to_string(r) + " names no variant");
+		}
+		g_table[TKey{static_cast<int>(x[0]), x[1], x[2], x[3]}] = static_cast<int>(x[4]);
+	}
+	return TTVX_OK;
 }
 
 int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags)
@@ -116,10 +178,20 @@ int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned f
 	Key const key{dt.code, f.O, f.K, f.I, align, f.qpos, f.p};
 	{
 		std::lock_guard<std::mutex> lock(g_sel_mutex);
-		w = g_weights;
-		if(!w) return -1;
 		auto it = g_cache.find(key);
 		if(it != g_cache.end()) return it->second;
+		// measured fastest variant of this folded problem, when the tuning table has it
+		if(f.three_d && align == 0) {
+			auto t = g_table.find(TKey{dt.code, f.O, f.K, f.I});
+			Config cfg;
+			if(t != g_table.end() && configure(t->second, f, dt, a_addr, 256, flags, cfg)) {
+				if(g_cache.size() > 4096) g_cache.clear();
+				g_cache[key] = t->second;
+				return t->second;
+			}
+		}
+		w = g_weights;
+		if(!w) return -1;
 	}
 	float x[kMlpF], y[kMlpV];
 	features_of(f, dt, a_addr, x);
@@ -147,6 +219,11 @@ int ttvx_selector_set_weights(const float* params, size_t count)
 {
 	ttvx::clear_error();
 	return ttvx::set_selector_weights(params, count);
+}
+int ttvx_selector_set_table(const int64_t* rows, size_t count)
+{
+	ttvx::clear_error();
+	return ttvx::set_selector_table(rows, count);
 }
 size_t ttvx_selector_num_params(void) { return ttvx::kMlpParams; }
 }

@@ -26,6 +26,7 @@ from .meta import generate_output_layout, generate_output_shape, generate_stride
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(HERE, "data", "selector_gfx950.npz")
 WEIGHTS_BIN = os.path.join(HERE, "data", "selector_gfx950.bin")     # raw float32, loaded by libttvx itself
+TABLE_BIN = os.path.join(HERE, "data", "selector_table_gfx950.bin")  # raw int64 rows, loaded by libttvx itself
 STRUCT_WEIGHT = 8     # the equal-extent benchmark shapes count this many times in the training loss
 NV = 13
 NF = 24
@@ -192,10 +193,11 @@ def shape_arrays(shapes, pmax=10):
 
 def shapes_from_arrays(z):
     out = []
-    for k in range(z["meta"].shape[0]):
-        p, q, code = (int(x) for x in z["meta"][k])
-        na = [int(x) for x in z["na"][k, :p]]
-        pia = [int(x) for x in z["pia"][k, :p]]
+    meta, na_all, pia_all = z["meta"], z["na"], z["pia"]      # read each (compressed) array once
+    for k in range(meta.shape[0]):
+        p, q, code = (int(x) for x in meta[k])
+        na = [int(x) for x in na_all[k, :p]]
+        pia = [int(x) for x in pia_all[k, :p]]
         out.append(dict(na=na, q=q, pia=pia, wa=generate_strides(na, pia), dtype=DT_NAME[code]))
     return out
 
@@ -311,6 +313,31 @@ def save(path, params, meta):
     params.astype("<f4").tofile(os.path.splitext(path)[0] + ".bin")
 
 
+def tuning_table(shapes, times):
+    """Rows (dtype, O, K, I, variant) of the measured fastest variant per folded problem; shapes
+    that fold to the same problem pool their timings (sum of slowdowns vs each one's fastest)."""
+    import ttv_amd
+    acc = {}
+    for s, t in zip(shapes, times):
+        if not np.isfinite(t).any():
+            continue
+        code = DT_CODE[s["dtype"]]
+        info = ttv_amd.plan_info(s["q"], s["na"], s["wa"], s["pia"], code)
+        if not info["three_d"]:
+            continue
+        key = (code, info["O"], info["K"], info["I"])
+        rel = np.where(np.isfinite(t), t / np.nanmin(t), np.inf)
+        acc[key] = acc.get(key, 0.0) + rel
+    rows = [list(k) + [int(np.argmin(v))] for k, v in sorted(acc.items())]
+    return np.array(rows, dtype=np.int64).reshape(-1, 5)
+
+
+def load_table(path=TABLE_BIN):
+    if not os.path.exists(path):
+        return None
+    return np.fromfile(path, dtype="<i8").reshape(-1, 5)
+
+
 def load_weights(path=WEIGHTS_PATH):
     if not os.path.exists(path):
         return None
@@ -384,6 +411,9 @@ def main():
             rep["regret_heuristic_test_" + name] = regret(h_te[sel], t_te[sel])
     log(json.dumps(rep))
     save(os.path.join(a.out, "selector_gfx950.npz"), params, dict(report=json.dumps(rep)))
+    # tuning table from the training shapes only (the held-out shapes stay unseen)
+    tshapes = shapes_from_arrays(ztr) if a.tables else shapes_tr
+    tuning_table(tshapes, t_tr).astype("<i8").tofile(os.path.join(a.out, "selector_table_gfx950.bin"))
     with open(os.path.join(a.out, "report.json"), "w") as f:
         json.dump(rep, f, indent=1)
 
