@@ -123,8 +123,13 @@ def valid_variants(shape):
     return [v for v in vs if v != STRIDED] or vs
 
 
-def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0):
-    """Median time (ms) of every valid variant (NaN for invalid ones)."""
+def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0, repeated=False):
+    """Median time (ms) of every valid variant (NaN for invalid ones).
+
+    Default: every launch reads a region no previous launch touched (a workload streaming
+    through many tensors).  repeated=True: every launch reads the same tensor, with the
+    last-level cache evicted before each launch (the reference's benchmark protocol: repeated
+    products over one tensor, README.md:50)."""
     import torch
     import ttv_amd
     elem = ELEM[shape["dtype"]]
@@ -133,25 +138,30 @@ def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0):
     vs = valid_variants(shape)
     out = np.full(NV, np.nan)
     stream = torch.cuda.current_stream()
+    fixed = pool.take(nbytes) if repeated else None
+    if repeated and getattr(pool, "flush", None) is None:
+        pool.flush = torch.empty(1 << 27, dtype=torch.float32, device=pool.A.device)   # 512 MiB
     # one untimed launch per variant (code objects resident, C allocated)
     for v in vs:
         ttv_amd.set_selection(ttv_amd.SELECT_FORCED, v)
-        run(pool.take(nbytes))
+        run(fixed if repeated else pool.take(nbytes))
     n_launch = len(vs) * reps
-    iters = int(min(100000, max(8, n_launch * spin_us_per_launch / 3.4)))
+    iters = int(min(100000, max(8, n_launch * (spin_us_per_launch + (100.0 if repeated else 0.0)) / 3.4)))
     check(lib().ttvx_bench_spin(iters, _ptr(pool.sink), _stream()))
     evs = []
     for v in vs:
         ttv_amd.set_selection(ttv_amd.SELECT_FORCED, v)
         for _ in range(reps):
-            off = pool.take(nbytes)
+            if repeated:
+                pool.flush.zero_()
+            off = fixed if repeated else pool.take(nbytes)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
             run(off)
             e1.record(stream)
             evs.append((v, e0, e1))
     torch.cuda.synchronize()
-    ttv_amd.set_selection(ttv_amd.SELECT_HEURISTIC)
+    ttv_amd.set_selection(ttv_amd.SELECT_AUTO)
     per = {}
     for v, e0, e1 in evs:
         per.setdefault(v, []).append(e0.elapsed_time(e1))
@@ -160,7 +170,7 @@ def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0):
     return out
 
 
-def collect_timings(shapes, reps=3, log=print, pool=None):
+def collect_timings(shapes, reps=3, log=print, pool=None, repeated=False):
     import ttv_amd
     pool = pool or TimingPool()
     feats = np.zeros((len(shapes), NF), np.float32)
@@ -171,7 +181,7 @@ def collect_timings(shapes, reps=3, log=print, pool=None):
         code = DT_CODE[s["dtype"]]
         feats[n] = ttv_amd.features(s["q"], s["na"], s["wa"], s["pia"], code)
         heur[n] = ttv_amd.plan_info(s["q"], s["na"], s["wa"], s["pia"], code)["variant"]
-        times[n] = time_shape(pool, s, reps=reps)
+        times[n] = time_shape(pool, s, reps=reps, repeated=repeated)
         if log and (n + 1) % 500 == 0:
             log("timed %d/%d shapes in %.0f s" % (n + 1, len(shapes), time.time() - t0))
     return feats, times, heur
@@ -359,9 +369,12 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default="gpurun_out/selector")
     ap.add_argument("--tables", help="directory with timings_train.npz / timings_test.npz to train on")
+    ap.add_argument("--retime-structured", action="store_true",
+                    help="with --tables: time the equal-extent shapes again (repeated-tensor protocol) first")
     a = ap.parse_args()
     import torch
     from .workloads import selector_shapes, structured_shapes
+    ztr = None
 
     def log(*x):
         print(*x, flush=True)
@@ -375,9 +388,17 @@ def main():
                     return np.load(os.path.join(a.tables, name))
             raise FileNotFoundError("no %s timing table under %s" % (kind, a.tables))
         ztr, zte = table("train"), table("test")
-        f_tr, t_tr, h_tr, n_struct = ztr["feats"], ztr["times"], ztr["heur"], int(ztr["n_structured"])
+        f_tr, t_tr, h_tr, n_struct = ztr["feats"], ztr["times"].copy(), ztr["heur"], int(ztr["n_structured"])
         f_te, t_te, h_te = zte["feats"], zte["times"], zte["heur"]
         n_tr = len(f_tr) - n_struct
+        if a.retime_structured:
+            st_shapes = shapes_from_arrays(ztr)[n_tr:]
+            _, t_st, _ = collect_timings(st_shapes, reps=5, log=log, repeated=True)
+            t_tr[n_tr:] = t_st
+            log("equal-extent shapes timed again in %.0f s" % (time.time() - t0))
+            np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
+                                n_structured=n_struct, **{k: ztr[k] for k in ("na", "pia", "meta")})
+            np.savez_compressed(os.path.join(a.out, "timings_test.npz"), **{k: zte[k] for k in zte.files})
     else:
         shapes_te = selector_shapes(seed=202, count=a.test)
         shapes_st = structured_shapes()
@@ -388,7 +409,9 @@ def main():
         log("held-out timings done in %.0f s" % (time.time() - t0))
         np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te,
                             **shape_arrays(shapes_te))
-        f_tr, t_tr, h_tr = collect_timings(shapes_tr, reps=a.reps, log=log, pool=pool)
+        f_r, t_r, h_r = collect_timings(shapes_tr[:n_tr], reps=a.reps, log=log, pool=pool)
+        f_s, t_s, h_s = collect_timings(shapes_tr[n_tr:], reps=5, log=log, pool=pool, repeated=True)
+        f_tr, t_tr, h_tr = np.concatenate([f_r, f_s]), np.concatenate([t_r, t_s]), np.concatenate([h_r, h_s])
         log("training timings done in %.0f s" % (time.time() - t0))
         np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
                             n_structured=n_struct, **shape_arrays(shapes_tr))
