@@ -291,8 +291,8 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
                                          typename Traits<S>::acc* red)
 {
 	using Acc = typename Traits<S>::acc;
-	// k unroll: U*KU independent 16-byte loads in flight per lane (8 for 4- and 8-byte elements,
-	// 4 for the narrow types, whose VEC accumulators already take the registers)
+	// k unroll: U*KU independent 16-byte loads in flight per lane (TTVX_KL_WIDE for 4- and 8-byte
+	// elements, TTVX_KL_NARROW for the 1- and 2-byte types)
 	constexpr int KL = sizeof(S) >= 4 ? TTVX_KL_WIDE : TTVX_KL_NARROW;
 	constexpr int KU = KL / U > 1 ? KL / U : 1;
 	int const t  = static_cast<int>(threadIdx.x);

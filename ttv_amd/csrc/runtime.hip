@@ -147,15 +147,22 @@ int pick_vec(const DType& dt, uintptr_t a_addr, int64_t inner, std::initializer_
 
 std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 
-// Work assignment of k_dot / k_col: 0 = grid-stride over up to kGridPerCU blocks per CU (measured
-// faster on MI355X for rows and narrow column tiles), 1 = one contiguous range per resident block.
+// Build-time tuning constants (the defaults are the values measured best on MI355X; the
+// benchmark sweeps rebuild with -D overrides to compare):
+//   TTVX_CONTIG_DEFAULT       work assignment of k_dot / k_col: 0 = grid-stride over up to
+//                             TTVX_GRID_PER_CU blocks per CU (faster for rows and narrow column
+//                             tiles), 1 = one contiguous range per block
+//   TTVX_GRID_PER_CU          block cap per CU of the column and strided launches
+//   TTVX_COL_WIDE_U           slab units per lane of the col_wide variant
+//   TTVX_DOT_GROUPS_PER_WAVE  row groups per wave the row launches aim at (twice that for 1- and
+//                             2-byte types, whose groups hold more rows)
 #ifndef TTVX_CONTIG_DEFAULT
 #define TTVX_CONTIG_DEFAULT 0
 #endif
 #ifndef TTVX_GRID_PER_CU
 #define TTVX_GRID_PER_CU 16
 #endif
-constexpr int kGridPerCU = TTVX_GRID_PER_CU;   // block cap (per CU) of the work loops
+constexpr int kGridPerCU = TTVX_GRID_PER_CU;
 #ifndef TTVX_COL_WIDE_U
 #define TTVX_COL_WIDE_U 4
 #endif
@@ -236,7 +243,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		// slab); the other column variants keep >= 16 k steps per lane and fill the block with slabs
 		int64_t const ks_cap = kBlock / P.IT;
 		int64_t ks_deep = std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K / 16));
-		if(is_pow2(P.IT)) ks_deep = int64_t{1} << ilog2(ks_deep);     // keeps IT*KS <= 64 on the butterfly
+		if(is_pow2(P.IT)) ks_deep = int64_t{1} << ilog2(ks_deep);     // power-of-two IT*KS <= 64 sums inside one wave
 		P.KS = static_cast<int>(v == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
 		static const int us[] = {1, 2, 4, 1, 1};
@@ -265,7 +272,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		cfg.grid   = static_cast<int>(std::max<int64_t>(1, std::min(P.items, gridcap)));
 	} else if(v == V_COL_K_LANES) {
 		if(!dot_shape) return false;
-		// the column kernel on a row problem: lanes along k (scalar loads), butterfly over 64 lanes
+		// the column kernel on a row problem: lanes along k (scalar loads), summed across 64 lanes
 		cfg.kind = KIND_COL;
 		cfg.vec  = 1;
 		P.I = 1; P.IT = 1;
