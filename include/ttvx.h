@@ -222,7 +222,8 @@ typedef struct ttvx_batch_problem {
  * returns an opaque handle. */
 int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** batch);
 /* Computes every problem of the batch in one persistent launch on `stream`; with use_graph != 0
- * the launch is captured into a hipGraph on first use and replayed afterwards. */
+ * the launch is captured into a hipGraph on first use and replayed afterwards.  A batch owns one
+ * tile counter: launches of the same batch must not overlap (one stream, or ordered streams). */
 int ttvx_batch_launch(void* batch, int use_graph, void* stream);
 /* out4 = {tiles, algorithmic bytes, persistent grid size, problems}. */
 int ttvx_batch_info(void* batch, int64_t* out4);

@@ -259,9 +259,20 @@ int ttvx_batch_launch(void* handle, int use_graph, void* stream)
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	if(!use_graph) return enqueue(*bt, s);
 	if(bt->graph_exec == nullptr) {
-		// capture once on a private stream (the legacy default stream cannot be captured)
-		hipError_t e = hipStreamCreateWithFlags(&bt->capture_stream, hipStreamNonBlocking);
-		if(e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
+		// capture once on a private stream (the legacy default stream cannot be captured); a failed
+		// earlier capture leaves the stream for this attempt and its graph is dropped
+		hipError_t e = hipSuccess;
+		if(bt->capture_stream == nullptr) {
+			e = hipStreamCreateWithFlags(&bt->capture_stream, hipStreamNonBlocking);
+			if(e != hipSuccess) {
+				bt->capture_stream = nullptr;
+				return hip_fail(e, "hipStreamCreateWithFlags");
+			}
+		}
+		if(bt->graph != nullptr) {
+			(void)hipGraphDestroy(bt->graph);
+			bt->graph = nullptr;
+		}
 		e = hipStreamBeginCapture(bt->capture_stream, hipStreamCaptureModeThreadLocal);
 		if(e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
 		int rc = enqueue(*bt, bt->capture_stream);
