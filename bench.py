@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the 16^7 FP32 and 256^3 FP64 cases")
+    ap.add_argument("--no-order8", action="store_true", help="skip the order-8 precision configuration")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile", action="store_true",
                     help="short fixed dispatch sequence for rocprofv3: copy and read streams, then each of the "
@@ -326,7 +327,74 @@ def main():
     if not args.no_extra:
         out["other_configs"] = other_configs_leg(torch, np, ttv_amd, Aflat, log)
 
+    if not args.no_order8:
+        del Aflat
+        for c in cases:
+            c.pop("A", None)
+            c.pop("C", None)
+        torch.cuda.empty_cache()
+        out["precision_order8"] = order8_leg(torch, np, ttv_amd, log)
+
     print(json.dumps(out), flush=True)
+
+
+def order8_leg(torch, np, ttv_amd, log, reps=5):
+    """The order-8 precision configuration: FP32 A of extents (4096,4,8,4,8,4,2,64) = 2^31 elements
+    (8 GiB; the product of the extents, not the ~1.07e9 sometimes quoted for it), layout drawn from
+    a seed, q in {1, 4, 8} (n_q = 4096, 4, 64); A ~ N(0,1) with 1% of entries x100, b ~ U[-1,1].
+    FP32, BF16 and FP8-e4m3 storage, FP32 accumulation: time (median of launches behind a spin,
+    cache evicted before each), speedup and its byte-bound ideal, relative L2 and max errors
+    against the FP64 oracle on the original values."""
+    from oracle import oracle as O
+    dev = torch.device("cuda:0")
+    na = [4096, 4, 8, 4, 8, 4, 2, 64]
+    rng = np.random.default_rng(31)
+    pia = [int(x) for x in rng.permutation(8) + 1]
+    wa = O.strides(na, pia)
+    n = int(np.prod(na))
+    g = torch.Generator(device=dev)
+    g.manual_seed(31)
+    A = torch.randn(n, generator=g, device=dev)
+    A.mul_(torch.where(torch.rand(n, generator=g, device=dev) < 0.01, 100.0, 1.0))
+    a = A.cpu().numpy()
+    Av = A.as_strided(na, wa)
+    Abf = ttv_amd.quantize_bf16(A).as_strided(na, wa)
+    sink = torch.zeros(1, dtype=torch.int32, device=dev)
+    flush = torch.empty(1 << 27, dtype=torch.float32, device=dev)
+    res = {"shape": na, "layout": pia, "elements": n, "cases": []}
+    for q in (1, 4, 8):
+        nq = na[q - 1]
+        b = rng.uniform(-1, 1, nq).astype(np.float32)
+        bd = torch.from_numpy(b).to(dev)
+        ref = O.naive(q, a, na, wa, b, pia=pia)
+        rn, rmax = float(np.linalg.norm(ref)), float(np.max(np.abs(ref)))
+        outc = n // nq
+        codes, scales, fps = ttv_amd.quantize_fp8(q, Av)
+        C32 = ttv_amd.ttv(q, Av, bd)
+        C16 = ttv_amd.ttv_bf16(q, Abf, bd)
+        C8 = ttv_amd.ttv_fp8(q, codes, scales, fps, bd)
+        runs = {"fp32": (lambda: ttv_amd.ttv(q, Av, bd, out=C32), C32, n * 4),
+                "bf16": (lambda: ttv_amd.ttv_bf16(q, Abf, bd, out=C16), C16, n * 2),
+                "fp8": (lambda: ttv_amd.ttv_fp8(q, codes, scales, fps, bd, out=C8), C8, n + scales.numel() * 4)}
+        case = {"q": q, "n_q": nq}
+        for k, (run, C, abytes) in runs.items():
+            ms = float(np.median(timed_launches(torch, ttv_amd, run, reps, sink, flush=flush)))
+            nbytes = abytes + outc * 4 + nq * 4
+            got = C.as_strided([C.numel()], [1]).cpu().numpy().astype(np.float64)
+            d = got - ref
+            case[k] = dict(ms=round(ms, 4), gbs=round(nbytes / (ms * 1e-3) / 1e9, 1), bytes=nbytes,
+                           rel_l2=float(np.linalg.norm(d) / rn), max_abs=float(np.max(np.abs(d))))
+        for k in ("bf16", "fp8"):
+            case[k]["speedup"] = round(case["fp32"]["ms"] / case[k]["ms"], 3)
+            case[k]["ideal_speedup"] = round(case["fp32"]["bytes"] / case[k]["bytes"], 3)
+        case["max_abs_C"] = rmax
+        res["cases"].append(case)
+        log("  order-8 q=%d (n_q=%d): fp32 %.3f ms %.0f GB/s | bf16 x%.2f (ideal %.2f) err %.2e | fp8 x%.2f (ideal %.2f) "
+            "err %.2e" % (q, nq, case["fp32"]["ms"], case["fp32"]["gbs"], case["bf16"]["speedup"],
+                          case["bf16"]["ideal_speedup"], case["bf16"]["rel_l2"], case["fp8"]["speedup"],
+                          case["fp8"]["ideal_speedup"], case["fp8"]["rel_l2"]))
+        del codes, scales, C32, C16, C8, ref
+    return res
 
 
 def host_staged_leg(np, ttv_amd, a_host, cases, log, reps=3):

@@ -163,9 +163,6 @@ std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 #define TTVX_GRID_PER_CU 16
 #endif
 constexpr int kGridPerCU = TTVX_GRID_PER_CU;
-#ifndef TTVX_KS_ONE_WAVE
-#define TTVX_KS_ONE_WAVE 0
-#endif
 #ifndef TTVX_COL_WIDE_U
 #define TTVX_COL_WIDE_U 4
 #endif
@@ -247,9 +244,6 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		int64_t const ks_cap = kBlock / P.IT;
 		int64_t ks_deep = std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K / 16));
 		if(is_pow2(P.IT)) ks_deep = int64_t{1} << ilog2(ks_deep);     // power-of-two IT*KS <= 64 sums inside one wave
-#if TTVX_KS_ONE_WAVE
-		if(P.IT < 64) ks_deep = std::min<int64_t>(ks_deep, std::max<int64_t>(1, 64 / P.IT));
-#endif
 		P.KS = static_cast<int>(v == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
 		static const int us[] = {1, 2, 4, 1, 1};
