@@ -199,6 +199,8 @@ struct Args {
 	int contig;
 	// column items with one k step per lane and no LDS reduction: double-buffered item loop
 	int one_step;
+	// bytes per vector store of C allowed by its alignment (0: element stores only)
+	int cvec;
 };
 
 // Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
@@ -262,6 +264,56 @@ __device__ __forceinline__ bool halving_sum(Acc (&v)[N], int lane, int stride, i
 	bool const store = d < 1 || ((lane / stride) & ((d << 1) - 1)) == 0;
 	for(; d >= 1; d >>= 1) v[0] += __shfl_xor(v[0], d * stride, 64);
 	return store;
+}
+
+// VEC consecutive outputs (o, i .. i+VEC-1) of one lane: vector stores when C's alignment allows
+// (P.cvec) and no split partials are pending; FP8 scales with one group-index division per vector.
+template<typename Acc, int VEC>
+__device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, int64_t o, int64_t i, const Acc (&v)[VEC])
+{
+	constexpr int BYTES = VEC * static_cast<int>(sizeof(Acc));
+	if(P.partial != nullptr || P.cvec == 0 || i + VEC > P.I || (BYTES >= 16 ? P.cvec < 16 : P.cvec < BYTES)) {
+#pragma unroll
+		for(int e = 0; e < VEC; ++e)
+			if(i + e < P.I) store_out<Acc>(C, P, o, i + e, v[e]);
+		return;
+	}
+	Acc w[VEC];
+#pragma unroll
+	for(int e = 0; e < VEC; ++e) w[e] = v[e];
+	if(P.scales != nullptr) {
+		int64_t const j0 = o * P.I + i;
+		int64_t const g0 = j0 / P.fps;
+		if(P.fps >= VEC) {
+			// at most one group boundary inside the vector
+			int64_t const rem = j0 - g0 * P.fps;
+			Acc const s0 = static_cast<Acc>(P.scales[g0]);
+			Acc const s1 = rem + VEC > P.fps ? static_cast<Acc>(P.scales[g0 + 1]) : s0;
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) w[e] *= rem + e < P.fps ? s0 : s1;
+		} else {
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) w[e] *= static_cast<Acc>(P.scales[(j0 + e) / P.fps]);
+		}
+	}
+	Acc* dst = static_cast<Acc*>(C) + o * P.sCo + i;
+	if constexpr(BYTES >= 16) {
+#pragma unroll
+		for(int c = 0; c < BYTES / 16; ++c) {
+			u32x4 q;
+			uint32_t const* src = reinterpret_cast<const uint32_t*>(w) + 4 * c;
+			q.x = src[0]; q.y = src[1]; q.z = src[2]; q.w = src[3];
+			reinterpret_cast<u32x4*>(dst)[c] = q;
+		}
+	} else if constexpr(BYTES == 8) {
+		u32x2 q;
+		uint32_t const* src = reinterpret_cast<const uint32_t*>(w);
+		q.x = src[0]; q.y = src[1];
+		*reinterpret_cast<u32x2*>(dst) = q;
+	} else {
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) dst[e] = w[e];
+	}
 }
 
 // ---- column tiles -------------------------------------------------------------------------
@@ -425,9 +477,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 		for(int u = 0; u < U; ++u) {
 			int64_t oo, ii;
 			col_unit(P, og, itile, os, it, u, U, VEC, oo, ii);
-			if(oo < P.O && ii < P.I)
-#pragma unroll
-				for(int e = 0; e < VEC; ++e) store_out<Acc>(C, P, oo, ii + e, v[u][e]);
+			if(oo < P.O && ii < P.I) store_vec<Acc, VEC>(C, P, oo, ii, v[u]);
 		}
 	}
 }

@@ -439,6 +439,17 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 			return fail(TTVX_ERR_UNSUPPORTED, "ttvx: no kernel variant can run this problem");
 	}
 	bool const kahan = (flags & TTVX_FLAG_KAHAN) != 0;
+	if(cfg.kind == KIND_COL) {
+		// largest power-of-two store (<= 16 bytes) that every lane's VEC outputs start aligned to
+		int64_t const ab = static_cast<int64_t>(acc_size(dt));
+		uintptr_t const c_addr = reinterpret_cast<uintptr_t>(c);
+		int chunk = 16;
+		while(chunk > static_cast<int>(ab) &&
+		      (c_addr % static_cast<uintptr_t>(chunk) != 0 || (f.O > 1 && (f.sCo * ab) % chunk != 0) ||
+		       (cfg.vec * ab) % chunk != 0))
+			chunk >>= 1;
+		cfg.args.cvec = chunk > static_cast<int>(ab) ? chunk : 0;
+	}
 
 	if(cfg.kind == KIND_STRIDED) {
 		cfg.sargs.scales = scales;
