@@ -159,3 +159,65 @@ def test_kahan_long_contraction(ttvx, device):
     e_plain, e_kahan = _rel(plain, ref), _rel(kahan, ref)
     assert e_kahan <= 1e-7, (e_plain, e_kahan)
     assert e_kahan <= e_plain
+
+
+@pytest.mark.parametrize("fmt", ["f32", "bf16", "fp8"])
+def test_output_alignment_padding_and_scale_groups(ttvx, device, fmt):
+    """Outputs written by vector stores: C at every 4-byte offset off the 16-byte grid, C with
+    padded rows, every kernel variant that can run, and FP8 scale groups smaller and larger than
+    a lane's vector (fibers_per_scale 1, 3, 16, 1000): each output equals the FP64 oracle on the
+    values the kernel reads."""
+    import torch
+    rng = np.random.default_rng(44)
+    na, pia, q = [64, 8, 6], [1, 2, 3], 2          # I = 64 columns (16-byte vectors), K = 8, O = 6
+    wa = O.strides(na, pia)
+    a = rng.uniform(-1, 1, O.span(na, wa)).astype(np.float32)
+    b = rng.uniform(-1, 1, na[q - 1]).astype(np.float32)
+    Ad = torch.from_numpy(a).to(device).as_strided(na, wa)
+    bd = torch.from_numpy(b).to(device)
+    nc = [64, 6]
+    if fmt == "f32":
+        ref = O.naive(q, a, na, wa, b, pia=pia)
+        groups = [None]
+    elif fmt == "bf16":
+        Aq = ttvx.quantize_bf16(Ad)
+        ref = O.naive(q, O.bf16_to_f32(O.bf16_rne(a)), na, wa, b, pia=pia)
+        groups = [None]
+    else:
+        groups = [1, 3, 16, 1000]
+    table = O.e4m3_decode(np.arange(256, dtype=np.uint8))
+    for fps in groups:
+        if fmt == "fp8":
+            codes, scales, fps = ttvx.quantize_fp8(q, Ad, fibers_per_scale=fps)
+            deq = table[codes.as_strided([a.size], [1]).cpu().numpy()]
+            ref = O.naive(q, deq, na, wa, b, pia=pia)
+            ref = ref * scales.cpu().numpy().astype(np.float64)[np.arange(ref.size) // fps]
+        for off, ld in ((0, 64), (1, 64), (2, 64), (3, 64), (0, 66), (1, 67)):
+            variants = [-1] + [v for v in range(13)
+                               if ttvx.variant_valid(q, na, wa, pia, v, {"f32": ttvx.F32, "bf16": ttvx.BF16,
+                                                                         "fp8": ttvx.FP8E4M3}[fmt])]
+            for v in variants:
+                buf = torch.full((off + ld * 6 + 8,), -7.0, dtype=torch.float32, device=device)
+                C = buf[off:].as_strided(nc, [1, ld])
+                if v < 0:
+                    ttvx.set_selection(ttvx.SELECT_AUTO)
+                else:
+                    ttvx.set_selection(ttvx.SELECT_FORCED, v)
+                try:
+                    if fmt == "f32":
+                        ttvx.ttv(q, Ad, bd, out=C)
+                    elif fmt == "bf16":
+                        ttvx.ttv_bf16(q, Aq, bd, out=C)
+                    else:
+                        ttvx.ttv_fp8(q, codes, scales, fps, bd, out=C)
+                finally:
+                    ttvx.set_selection(ttvx.SELECT_AUTO)
+                torch.cuda.synchronize()
+                got = buf.cpu().numpy()
+                mat = np.stack([got[off + o * ld: off + o * ld + 64] for o in range(6)])   # [o][i]
+                assert _rel(mat.reshape(-1), ref) <= 1e-5, (fmt, fps, off, ld, v)
+                # nothing written outside the outputs (padding and the bytes before C keep -7)
+                mask = np.ones(got.size, bool)
+                for o in range(6):
+                    mask[off + o * ld: off + o * ld + 64] = False
+                assert np.all(got[mask] == -7.0), (fmt, fps, off, ld, v)
