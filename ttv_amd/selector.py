@@ -170,6 +170,16 @@ def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0, repeated=False):
     return out
 
 
+def fixed_rule_variant(shape):
+    """The variant the fixed rule picks (whatever the process's current selection mode)."""
+    import ttv_amd
+    ttv_amd.set_selection(ttv_amd.SELECT_HEURISTIC)
+    try:
+        return ttv_amd.plan_info(shape["q"], shape["na"], shape["wa"], shape["pia"], DT_CODE[shape["dtype"]])["variant"]
+    finally:
+        ttv_amd.set_selection(ttv_amd.SELECT_AUTO)
+
+
 def collect_timings(shapes, reps=3, log=print, pool=None, repeated=False):
     import ttv_amd
     pool = pool or TimingPool()
@@ -180,7 +190,7 @@ def collect_timings(shapes, reps=3, log=print, pool=None, repeated=False):
     for n, s in enumerate(shapes):
         code = DT_CODE[s["dtype"]]
         feats[n] = ttv_amd.features(s["q"], s["na"], s["wa"], s["pia"], code)
-        heur[n] = ttv_amd.plan_info(s["q"], s["na"], s["wa"], s["pia"], code)["variant"]
+        heur[n] = fixed_rule_variant(s)
         times[n] = time_shape(pool, s, reps=reps, repeated=repeated)
         if log and (n + 1) % 500 == 0:
             log("timed %d/%d shapes in %.0f s" % (n + 1, len(shapes), time.time() - t0))
