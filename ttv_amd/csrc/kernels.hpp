@@ -218,7 +218,11 @@ __device__ __forceinline__ void block_range(int contig, int64_t n, int64_t& begi
 	}
 }
 
-template<typename Acc>
+// SC: the storage type carries per-group scales (FP8); for the other types the scale code is not
+// generated at all.
+template<typename S> constexpr bool kScaled = std::is_same_v<S, uint8_t>;
+
+template<typename Acc, bool SC>
 __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, int64_t o, int64_t i, Acc v)
 {
 	int64_t const j = o * P.I + i;
@@ -228,7 +232,8 @@ __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, i
 		else         static_cast<Acc*>(P.partial)[off]    = v;
 		return;
 	}
-	if(P.scales != nullptr) v *= static_cast<Acc>(P.scales[j / P.fps]);
+	if constexpr(SC)
+		if(P.scales != nullptr) v *= static_cast<Acc>(P.scales[j / P.fps]);
 	static_cast<Acc*>(C)[o * P.sCo + i] = v;
 }
 
@@ -268,20 +273,20 @@ __device__ __forceinline__ bool halving_sum(Acc (&v)[N], int lane, int stride, i
 
 // VEC consecutive outputs (o, i .. i+VEC-1) of one lane: vector stores when C's alignment allows
 // (P.cvec) and no split partials are pending; FP8 scales with one group-index division per vector.
-template<typename Acc, int VEC>
+template<typename Acc, int VEC, bool SC>
 __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, int64_t o, int64_t i, const Acc (&v)[VEC])
 {
 	constexpr int BYTES = VEC * static_cast<int>(sizeof(Acc));
 	if(P.partial != nullptr || P.cvec == 0 || i + VEC > P.I || (BYTES >= 16 ? P.cvec < 16 : P.cvec < BYTES)) {
 #pragma unroll
 		for(int e = 0; e < VEC; ++e)
-			if(i + e < P.I) store_out<Acc>(C, P, o, i + e, v[e]);
+			if(i + e < P.I) store_out<Acc, SC>(C, P, o, i + e, v[e]);
 		return;
 	}
 	Acc w[VEC];
 #pragma unroll
 	for(int e = 0; e < VEC; ++e) w[e] = v[e];
-	if(P.scales != nullptr) {
+	if(SC && P.scales != nullptr) {
 		int64_t const j0 = o * P.I + i;
 		int64_t const g0 = j0 / P.fps;
 		if(P.fps >= VEC) {
@@ -316,6 +321,23 @@ __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, i
 	}
 }
 
+// item -> (slab group, i tile): no division for a single i tile, 32-bit division when it fits
+__device__ __forceinline__ void item_split(const Args& P, int64_t item, int64_t& og, int64_t& itile)
+{
+	if(P.iTiles == 1) {
+		og = item;
+		itile = 0;
+	} else if(P.items <= 0xffffffffll) {
+		uint32_t const it32 = static_cast<uint32_t>(item), n32 = static_cast<uint32_t>(P.iTiles);
+		uint32_t const q = it32 / n32;
+		og = q;
+		itile = it32 - q * n32;
+	} else {
+		og = item / P.iTiles;
+		itile = item - og * P.iTiles;
+	}
+}
+
 // ---- column tiles -------------------------------------------------------------------------
 // Block tile: OS*U output slabs x IT*VEC columns (or OS slabs x U*IT*VEC columns when the U
 // units lie along i), all of the block's k range.
@@ -333,7 +355,7 @@ __device__ __forceinline__ void col_unit(const Args& P, int64_t og, int64_t itil
 		i = (itile * P.IT + it) * VEC;
 	}
 }
-template<typename Acc, int VEC, int U>
+template<typename Acc, int VEC, int U, bool SC>
 __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, int64_t og, int64_t itile, int os,
                                           int it, int ks, int t, bool active, Acc (&v)[U][VEC], Acc* red);
 
@@ -354,8 +376,8 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 	bool const active = os < P.OS;
 	int64_t const kstep = static_cast<int64_t>(P.KS) * P.sAk;
 
-	int64_t const og    = item / P.iTiles;
-	int64_t const itile = item - og * P.iTiles;
+	int64_t og, itile;
+	item_split(P, item, og, itile);
 	int64_t o[U], iu[U];
 	const S* pa[U];
 #pragma unroll
@@ -414,12 +436,12 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 	for(int u = 0; u < U; ++u)
 #pragma unroll
 		for(int e = 0; e < VEC; ++e) v[u][e] = acc[u][e].value();
-	col_store<Acc, VEC, U>(C, P, og, itile, os, it, ks, t, active, v, red);
+	col_store<Acc, VEC, U, kScaled<S>>(C, P, og, itile, os, it, ks, t, active, v, red);
 }
 
 // Sums the lanes' partial sums of one column item over its KS lane groups and stores the item's
 // outputs (shuffles inside a wave, or LDS across waves).
-template<typename Acc, int VEC, int U>
+template<typename Acc, int VEC, int U, bool SC>
 __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, int64_t og, int64_t itile, int os,
                                           int it, int ks, int t, bool active, Acc (&v)[U][VEC], Acc* red)
 {
@@ -440,7 +462,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 					int64_t oo, ii;
 					col_unit(P, og, itile, os, it, f / VEC, U, VEC, oo, ii);
 					ii += f % VEC;
-					if(oo < P.O && ii < P.I) store_out<Acc>(C, P, oo, ii, w[j]);
+					if(oo < P.O && ii < P.I) store_out<Acc, SC>(C, P, oo, ii, w[j]);
 				}
 		}
 		return;
@@ -466,7 +488,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 			int64_t oo, ii;
 			col_unit(P, og, itile, orr, ir, ur, U, VEC, oo, ii);
 			ii += e;
-			if(oo < P.O && ii < P.I) store_out<Acc>(C, P, oo, ii, s);
+			if(oo < P.O && ii < P.I) store_out<Acc, SC>(C, P, oo, ii, s);
 		}
 		__syncthreads();
 		return;
@@ -477,7 +499,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 		for(int u = 0; u < U; ++u) {
 			int64_t oo, ii;
 			col_unit(P, og, itile, os, it, u, U, VEC, oo, ii);
-			if(oo < P.O && ii < P.I) store_vec<Acc, VEC>(C, P, oo, ii, v[u]);
+			if(oo < P.O && ii < P.I) store_vec<Acc, VEC, SC>(C, P, oo, ii, v[u]);
 		}
 	}
 }
@@ -500,7 +522,8 @@ __device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, con
 	bool const live = active && k < ke;          // lanes past the end of the contraction add zero
 	Acc const bk = live ? b[k] : Acc(0);
 	auto load = [&](int64_t item, Vec<S, VEC> (&raw)[U]) {
-		int64_t const og = item / P.iTiles, itile = item - og * P.iTiles;
+		int64_t og, itile;
+		item_split(P, item, og, itile);
 #pragma unroll
 		for(int u = 0; u < U; ++u) {
 			int64_t o, i;
@@ -514,7 +537,8 @@ __device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, con
 		}
 	};
 	auto finish = [&](int64_t item, const Vec<S, VEC> (&raw)[U]) {
-		int64_t const og = item / P.iTiles, itile = item - og * P.iTiles;
+		int64_t og, itile;
+		item_split(P, item, og, itile);
 		Acc v[U][VEC];
 #pragma unroll
 		for(int u = 0; u < U; ++u) {
@@ -523,7 +547,7 @@ __device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, con
 #pragma unroll
 			for(int e = 0; e < VEC; ++e) v[u][e] = x[e] * bk;
 		}
-		col_store<Acc, VEC, U>(C, P, og, itile, os, it, ks, t, active, v, nullptr);
+		col_store<Acc, VEC, U, kScaled<S>>(C, P, og, itile, os, it, ks, t, active, v, nullptr);
 	};
 	Vec<S, VEC> ra[U], rb[U];
 	if(ib < ie) load(ib, ra);
@@ -580,7 +604,7 @@ __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __rest
 }
 
 // sums the lanes' partial row sums of group g and stores the U*RPW rows
-template<typename Acc, int U>
+template<typename Acc, int U, bool SC>
 __device__ __forceinline__ void dot_store(void* __restrict__ C, const Args& P, int64_t g, Acc (&v)[U])
 {
 	int const lane = static_cast<int>(threadIdx.x) & 63;
@@ -592,7 +616,7 @@ __device__ __forceinline__ void dot_store(void* __restrict__ C, const Args& P, i
 		for(int j = 0; j < U; ++j)
 			if(j < n) {
 				int64_t const row = (g * U + base + j) * rpw + r;
-				if(row < P.O) store_out<Acc>(C, P, row, 0, v[j]);
+				if(row < P.O) store_out<Acc, SC>(C, P, row, 0, v[j]);
 			}
 	}
 }
@@ -639,7 +663,7 @@ __device__ __forceinline__ void dot_rows_pipelined(const S* __restrict__ A, void
 			for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
 			v[u] = acc.value();
 		}
-		dot_store<Acc, U>(C, P, g, v);
+		dot_store<Acc, U, kScaled<S>>(C, P, g, v);
 	};
 	Vec<S, VEC> ra[U], rb[U];
 	if(g0 < ge) load(g0, ra);
@@ -686,7 +710,7 @@ __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* _
 	Acc v[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-	dot_store<Acc, U>(C, P, g, v);
+	dot_store<Acc, U, kScaled<S>>(C, P, g, v);
 }
 
 // long rows: b is re-read through the cache, loads of U rows and two k steps overlap
@@ -743,7 +767,7 @@ __device__ __forceinline__ void dot_group_long(const S* __restrict__ A, const ty
 	Acc v[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-	dot_store<Acc, U>(C, P, g, v);
+	dot_store<Acc, U, kScaled<S>>(C, P, g, v);
 }
 
 template<typename S, int VEC, int U, bool KAHAN>
