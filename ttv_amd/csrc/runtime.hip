@@ -163,6 +163,9 @@ std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 #define TTVX_GRID_PER_CU 16
 #endif
 constexpr int kGridPerCU = TTVX_GRID_PER_CU;
+#ifndef TTVX_MIN_ROW_BYTES
+#define TTVX_MIN_ROW_BYTES 0
+#endif
 #ifndef TTVX_COL_WIDE_U
 #define TTVX_COL_WIDE_U 4
 #endif
@@ -243,6 +246,14 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		// slab); the other column variants keep >= 16 k steps per lane and fill the block with slabs
 		int64_t const ks_cap = kBlock / P.IT;
 		int64_t ks_deep = std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K / 16));
+#if TTVX_MIN_ROW_BYTES
+		{
+			// rows narrower than a cache line: enough k rows per wave instruction to fill whole lines
+			int64_t const row = static_cast<int64_t>(P.IT) * cfg.vec * dt.size;
+			if(row < TTVX_MIN_ROW_BYTES)
+				ks_deep = std::max<int64_t>(ks_deep, std::min<int64_t>(ks_cap, std::min<int64_t>(f.K, TTVX_MIN_ROW_BYTES / row)));
+		}
+#endif
 		if(is_pow2(P.IT)) ks_deep = int64_t{1} << ilog2(ks_deep);     // power-of-two IT*KS <= 64 sums inside one wave
 		P.KS = static_cast<int>(v == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
