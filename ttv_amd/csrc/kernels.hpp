@@ -58,6 +58,7 @@ template<> struct Traits<uint8_t>  { using acc = float;  static constexpr int vm
 // A is device memory: the loads go through the global (not the flat) path, so they only count
 // against vmcnt.
 #define TTVX_GLOBAL(T, p) ((const __attribute__((address_space(1))) T*)(p))
+#define TTVX_GLOBAL_W(T, p) ((__attribute__((address_space(1))) T*)(p))
 template<int BYTES> struct RawLoad;
 template<> struct RawLoad<16> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[4]) {
@@ -222,7 +223,11 @@ __device__ __forceinline__ void block_range(int contig, int64_t n, int64_t& begi
 // generated at all.
 template<typename S> constexpr bool kScaled = std::is_same_v<S, uint8_t>;
 
-template<typename Acc, bool SC>
+// NT: the final store of C is non-temporal.  Chosen per store site at compile time: it pays where one
+// wave instruction writes whole 128-byte lines (+4..12 % on outputs of 1/16..1/4 of A) and costs
+// where lanes write pieces of lines that other instructions complete (up to 2x slower), because
+// non-temporal stores are not combined in L2.  Split partials are re-read and stay cached.
+template<typename Acc, bool SC, bool NT = false>
 __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, int64_t o, int64_t i, Acc v)
 {
 	int64_t const j = o * P.I + i;
@@ -234,7 +239,9 @@ __device__ __forceinline__ void store_out(void* __restrict__ C, const Args& P, i
 	}
 	if constexpr(SC)
 		if(P.scales != nullptr) v *= static_cast<Acc>(P.scales[j / P.fps]);
-	static_cast<Acc*>(C)[o * P.sCo + i] = v;
+	Acc* const dst = static_cast<Acc*>(C) + (o * P.sCo + i);
+	if constexpr(NT) __builtin_nontemporal_store(v, TTVX_GLOBAL_W(Acc, dst));
+	else             *dst = v;
 }
 
 // ---- cross-lane sums ------------------------------------------------------------------------
@@ -273,6 +280,8 @@ __device__ __forceinline__ bool halving_sum(Acc (&v)[N], int lane, int stride, i
 
 // VEC consecutive outputs (o, i .. i+VEC-1) of one lane: vector stores when C's alignment allows
 // (P.cvec) and no split partials are pending; FP8 scales with one group-index division per vector.
+// When a lane's vector is exactly one 16-byte store, consecutive lanes write consecutive 16-byte
+// pieces and the store is non-temporal; wider vectors (BF16, FP8) take several interleaved stores.
 template<typename Acc, int VEC, bool SC>
 __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, int64_t o, int64_t i, const Acc (&v)[VEC])
 {
@@ -302,7 +311,12 @@ __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, i
 		}
 	}
 	Acc* dst = static_cast<Acc*>(C) + o * P.sCo + i;
-	if constexpr(BYTES >= 16) {
+	if constexpr(BYTES == 16) {
+		u32x4 q;
+		uint32_t const* src = reinterpret_cast<const uint32_t*>(w);
+		q.x = src[0]; q.y = src[1]; q.z = src[2]; q.w = src[3];
+		__builtin_nontemporal_store(q, TTVX_GLOBAL_W(u32x4, dst));
+	} else if constexpr(BYTES > 16) {
 #pragma unroll
 		for(int c = 0; c < BYTES / 16; ++c) {
 			u32x4 q;
@@ -488,7 +502,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 			int64_t oo, ii;
 			col_unit(P, og, itile, orr, ir, ur, U, VEC, oo, ii);
 			ii += e;
-			if(oo < P.O && ii < P.I) store_out<Acc, SC>(C, P, oo, ii, s);
+			if(oo < P.O && ii < P.I) store_out<Acc, SC, true>(C, P, oo, ii, s);
 		}
 		__syncthreads();
 		return;
@@ -603,8 +617,9 @@ __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __rest
 		}
 }
 
-// sums the lanes' partial row sums of group g and stores the U*RPW rows
-template<typename Acc, int U, bool SC>
+// sums the lanes' partial row sums of group g and stores the U*RPW rows (non-temporal for 4- and
+// 8-byte storage, whose short rows leave whole runs of rows to one store instruction)
+template<typename Acc, int U, bool SC, bool NT>
 __device__ __forceinline__ void dot_store(void* __restrict__ C, const Args& P, int64_t g, Acc (&v)[U])
 {
 	int const lane = static_cast<int>(threadIdx.x) & 63;
@@ -616,7 +631,7 @@ __device__ __forceinline__ void dot_store(void* __restrict__ C, const Args& P, i
 		for(int j = 0; j < U; ++j)
 			if(j < n) {
 				int64_t const row = (g * U + base + j) * rpw + r;
-				if(row < P.O) store_out<Acc, SC>(C, P, row, 0, v[j]);
+				if(row < P.O) store_out<Acc, SC, NT>(C, P, row, 0, v[j]);
 			}
 	}
 }
@@ -663,7 +678,7 @@ __device__ __forceinline__ void dot_rows_pipelined(const S* __restrict__ A, void
 			for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
 			v[u] = acc.value();
 		}
-		dot_store<Acc, U, kScaled<S>>(C, P, g, v);
+		dot_store<Acc, U, kScaled<S>, (sizeof(S) >= 4)>(C, P, g, v);
 	};
 	Vec<S, VEC> ra[U], rb[U];
 	if(g0 < ge) load(g0, ra);
@@ -710,7 +725,7 @@ __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* _
 	Acc v[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-	dot_store<Acc, U, kScaled<S>>(C, P, g, v);
+	dot_store<Acc, U, kScaled<S>, (sizeof(S) >= 4)>(C, P, g, v);
 }
 
 // long rows: b is re-read through the cache, loads of U rows and two k steps overlap
@@ -767,7 +782,7 @@ __device__ __forceinline__ void dot_group_long(const S* __restrict__ A, const ty
 	Acc v[U];
 #pragma unroll
 	for(int u = 0; u < U; ++u) v[u] = acc[u].value();
-	dot_store<Acc, U, kScaled<S>>(C, P, g, v);
+	dot_store<Acc, U, kScaled<S>, (sizeof(S) >= 4)>(C, P, g, v);
 }
 
 template<typename S, int VEC, int U, bool KAHAN>

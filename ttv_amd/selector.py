@@ -67,6 +67,16 @@ class TimingPool:
         self.sink = torch.zeros(1, dtype=torch.int32, device=dev)
         self.offset = 0
 
+    @classmethod
+    def for_shapes(cls, shapes, seed=0):
+        """A pool large enough for every shape's tensor (at least 2 GiB) and output."""
+        a_bytes, c_words = 2 << 30, 1 << 28
+        for s in shapes:
+            n = int(np.prod(s["na"], dtype=np.int64))
+            a_bytes = max(a_bytes, n * ELEM[s["dtype"]])
+            c_words = max(c_words, n // s["na"][s["q"] - 1] * (2 if s["dtype"] == "f64" else 1))
+        return cls(pool_bytes=(a_bytes + 255) // 256 * 256, c_elems=c_words, seed=seed)
+
     def take(self, nbytes):
         """Byte offset of a region of nbytes not used by the previous launches (256-byte aligned)."""
         nbytes = (nbytes + 255) // 256 * 256
@@ -182,7 +192,7 @@ def fixed_rule_variant(shape):
 
 def collect_timings(shapes, reps=3, log=print, pool=None, repeated=False):
     import ttv_amd
-    pool = pool or TimingPool()
+    pool = pool or TimingPool.for_shapes(shapes)
     feats = np.zeros((len(shapes), NF), np.float32)
     times = np.full((len(shapes), NV), np.nan)
     heur = np.zeros(len(shapes), np.int32)
@@ -383,7 +393,7 @@ def main():
                     help="with --tables: time the equal-extent shapes again (repeated-tensor protocol) first")
     a = ap.parse_args()
     import torch
-    from .workloads import selector_shapes, structured_shapes
+    from .workloads import order8_shapes, selector_shapes, structured_shapes
     ztr = None
 
     def log(*x):
@@ -411,10 +421,10 @@ def main():
             np.savez_compressed(os.path.join(a.out, "timings_test.npz"), **{k: zte[k] for k in zte.files})
     else:
         shapes_te = selector_shapes(seed=202, count=a.test)
-        shapes_st = structured_shapes()
+        shapes_st = structured_shapes() + order8_shapes()
         shapes_tr = selector_shapes(seed=101, count=a.train) + shapes_st
         n_tr, n_struct = a.train, len(shapes_st)
-        pool = TimingPool()
+        pool = TimingPool.for_shapes(shapes_tr)
         f_te, t_te, h_te = collect_timings(shapes_te, reps=a.reps, log=log, pool=pool)
         log("held-out timings done in %.0f s" % (time.time() - t0))
         np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te,

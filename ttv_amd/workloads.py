@@ -45,6 +45,22 @@ def structured_shapes(dtypes=("f32", "bf16", "f64", "fp8"), sizes_log2=(20, 22, 
     return out
 
 
+ORDER8 = [4096, 4, 8, 4, 8, 4, 2, 64]
+
+
+def order8_shapes(dtypes=("f32", "bf16", "f64", "fp8")):
+    """The order-8 configuration (extents 4096,4,8,4,8,4,2,64 = 2^31 elements) for every q in
+    first-order, last-order and the layout that the benchmark draws (permutation of seed 31),
+    every storage type."""
+    seeded = [int(x) for x in np.random.default_rng(31).permutation(8) + 1]
+    out = []
+    for pia in (list(range(1, 9)), list(range(8, 0, -1)), seeded):
+        for q in range(1, 9):
+            for dt in dtypes:
+                out.append(dict(na=list(ORDER8), q=q, pia=pia, wa=generate_strides(ORDER8, pia), dtype=dt))
+    return out
+
+
 def selector_shapes(seed, count, lo_log2=20, hi_log2=28, pmin=2, pmax=10, dtypes=("f32", "bf16", "f64", "fp8")):
     rng = np.random.default_rng(seed)
     out = []
