@@ -27,7 +27,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(HERE, "data", "selector_gfx950.npz")
 WEIGHTS_BIN = os.path.join(HERE, "data", "selector_gfx950.bin")     # raw float32, loaded by libttvx itself
 TABLE_BIN = os.path.join(HERE, "data", "selector_table_gfx950.bin")  # raw int64 rows, loaded by libttvx itself
-STRUCT_WEIGHT = 8     # the equal-extent benchmark shapes count this many times in the training loss
+STRUCT_WEIGHT = 8     # the benchmark shapes (equal extents, order 8) count this many times in the training loss
 NV = 13
 NF = 24
 DT_CODE = {"f32": _lib.F32, "bf16": _lib.BF16, "f64": _lib.F64, "fp8": _lib.FP8E4M3}
@@ -289,7 +289,7 @@ class GPUTrainer:
 
 
 def train(feats, times, epochs=300, batch=512, lr=2e-3, seed=0, log=None, n_structured=0, struct_weight=1):
-    """Adam on the GPU kernels; the last n_structured rows (equal-extent shapes) are repeated
+    """Adam on the GPU kernels; the last n_structured rows (benchmark shapes) are repeated
     struct_weight times, so the common benchmark shapes weigh more than one random shape each."""
     import torch
     dev = torch.device("cuda:0")
@@ -390,7 +390,7 @@ def main():
     ap.add_argument("--out", default="gpurun_out/selector")
     ap.add_argument("--tables", help="directory with timings_train.npz / timings_test.npz to train on")
     ap.add_argument("--retime-structured", action="store_true",
-                    help="with --tables: time the equal-extent shapes again (repeated-tensor protocol) first")
+                    help="with --tables: time the benchmark shapes again (repeated-tensor protocol) first")
     a = ap.parse_args()
     import torch
     from .workloads import order8_shapes, selector_shapes, structured_shapes
@@ -415,7 +415,7 @@ def main():
             st_shapes = shapes_from_arrays(ztr)[n_tr:]
             _, t_st, _ = collect_timings(st_shapes, reps=5, log=log, repeated=True)
             t_tr[n_tr:] = t_st
-            log("equal-extent shapes timed again in %.0f s" % (time.time() - t0))
+            log("benchmark shapes timed again in %.0f s" % (time.time() - t0))
             np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
                                 n_structured=n_struct, **{k: ztr[k] for k in ("na", "pia", "meta")})
             np.savez_compressed(os.path.join(a.out, "timings_test.npz"), **{k: zte[k] for k in zte.files})
