@@ -161,21 +161,28 @@ def test_kahan_long_contraction(ttvx, device):
     assert e_kahan <= e_plain
 
 
+@pytest.mark.parametrize("rows", ["narrow", "wide"])
 @pytest.mark.parametrize("fmt", ["f32", "bf16", "fp8"])
-def test_output_alignment_padding_and_scale_groups(ttvx, device, fmt):
+def test_output_alignment_padding_and_scale_groups(ttvx, device, fmt, rows):
     """Outputs written by vector stores: C at every 4-byte offset off the 16-byte grid, C with
     padded rows, every kernel variant that can run, and FP8 scale groups smaller and larger than
     a lane's vector (fibers_per_scale 1, 3, 16, 1000): each output equals the FP64 oracle on the
-    values the kernel reads."""
+    values the kernel reads.  "wide": 300 vectors per output row, so whole waves lie along i
+    (the wave-transposed stores of BF16/FP8) and the last tile of 256 lanes is partly past the row."""
     import torch
     rng = np.random.default_rng(44)
-    na, pia, q = [64, 8, 6], [1, 2, 3], 2          # I = 64 columns (16-byte vectors), K = 8, O = 6
+    vec = {"f32": 4, "bf16": 8, "fp8": 16}[fmt]
+    if rows == "narrow":
+        na, pia, q = [64, 8, 6], [1, 2, 3], 2      # I = 64 columns (16-byte vectors), K = 8, O = 6
+    else:
+        na, pia, q = [300 * vec, 4, 5], [1, 2, 3], 2
+    ni, no = na[0], na[2]
     wa = O.strides(na, pia)
     a = rng.uniform(-1, 1, O.span(na, wa)).astype(np.float32)
     b = rng.uniform(-1, 1, na[q - 1]).astype(np.float32)
     Ad = torch.from_numpy(a).to(device).as_strided(na, wa)
     bd = torch.from_numpy(b).to(device)
-    nc = [64, 6]
+    nc = [ni, no]
     if fmt == "f32":
         ref = O.naive(q, a, na, wa, b, pia=pia)
         groups = [None]
@@ -192,12 +199,12 @@ def test_output_alignment_padding_and_scale_groups(ttvx, device, fmt):
             deq = table[codes.as_strided([a.size], [1]).cpu().numpy()]
             ref = O.naive(q, deq, na, wa, b, pia=pia)
             ref = ref * scales.cpu().numpy().astype(np.float64)[np.arange(ref.size) // fps]
-        for off, ld in ((0, 64), (1, 64), (2, 64), (3, 64), (0, 66), (1, 67)):
+        for off, ld in ((0, ni), (1, ni), (2, ni), (3, ni), (0, ni + 2), (1, ni + 3)):
             variants = [-1] + [v for v in range(13)
                                if ttvx.variant_valid(q, na, wa, pia, v, {"f32": ttvx.F32, "bf16": ttvx.BF16,
                                                                          "fp8": ttvx.FP8E4M3}[fmt])]
             for v in variants:
-                buf = torch.full((off + ld * 6 + 8,), -7.0, dtype=torch.float32, device=device)
+                buf = torch.full((off + ld * no + 8,), -7.0, dtype=torch.float32, device=device)
                 C = buf[off:].as_strided(nc, [1, ld])
                 if v < 0:
                     ttvx.set_selection(ttvx.SELECT_AUTO)
@@ -214,10 +221,10 @@ def test_output_alignment_padding_and_scale_groups(ttvx, device, fmt):
                     ttvx.set_selection(ttvx.SELECT_AUTO)
                 torch.cuda.synchronize()
                 got = buf.cpu().numpy()
-                mat = np.stack([got[off + o * ld: off + o * ld + 64] for o in range(6)])   # [o][i]
+                mat = np.stack([got[off + o * ld: off + o * ld + ni] for o in range(no)])   # [o][i]
                 assert _rel(mat.reshape(-1), ref) <= 1e-5, (fmt, fps, off, ld, v)
                 # nothing written outside the outputs (padding and the bytes before C keep -7)
                 mask = np.ones(got.size, bool)
-                for o in range(6):
-                    mask[off + o * ld: off + o * ld + 64] = False
+                for o in range(no):
+                    mask[off + o * ld: off + o * ld + ni] = False
                 assert np.all(got[mask] == -7.0), (fmt, fps, off, ld, v)

@@ -202,6 +202,9 @@ struct Args {
 	int one_step;
 	// bytes per vector store of C allowed by its alignment (0: element stores only)
 	int cvec;
+	// column outputs without reduction whose lane vectors exceed 16 bytes (BF16, FP8), a whole wave
+	// along i and 16-byte aligned C: stored wave-transposed (col_store_transposed)
+	int tstore;
 };
 
 // Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
@@ -278,6 +281,26 @@ __device__ __forceinline__ bool halving_sum(Acc (&v)[N], int lane, int stride, i
 	return store;
 }
 
+// FP8 scales of the VEC consecutive outputs (o, i .. i+VEC-1), one group-index division per vector
+template<typename Acc, int VEC>
+__device__ __forceinline__ void scale_vec(const Args& P, int64_t o, int64_t i, Acc (&w)[VEC])
+{
+	if(P.scales == nullptr) return;
+	int64_t const j0 = o * P.I + i;
+	int64_t const g0 = j0 / P.fps;
+	if(P.fps >= VEC) {
+		// at most one group boundary inside the vector
+		int64_t const rem = j0 - g0 * P.fps;
+		Acc const s0 = static_cast<Acc>(P.scales[g0]);
+		Acc const s1 = rem + VEC > P.fps ? static_cast<Acc>(P.scales[g0 + 1]) : s0;
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) w[e] *= rem + e < P.fps ? s0 : s1;
+	} else {
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) w[e] *= static_cast<Acc>(P.scales[(j0 + e) / P.fps]);
+	}
+}
+
 // VEC consecutive outputs (o, i .. i+VEC-1) of one lane: vector stores when C's alignment allows
 // (P.cvec) and no split partials are pending; FP8 scales with one group-index division per vector.
 // When a lane's vector is exactly one 16-byte store, consecutive lanes write consecutive 16-byte
@@ -295,21 +318,7 @@ __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, i
 	Acc w[VEC];
 #pragma unroll
 	for(int e = 0; e < VEC; ++e) w[e] = v[e];
-	if(SC && P.scales != nullptr) {
-		int64_t const j0 = o * P.I + i;
-		int64_t const g0 = j0 / P.fps;
-		if(P.fps >= VEC) {
-			// at most one group boundary inside the vector
-			int64_t const rem = j0 - g0 * P.fps;
-			Acc const s0 = static_cast<Acc>(P.scales[g0]);
-			Acc const s1 = rem + VEC > P.fps ? static_cast<Acc>(P.scales[g0 + 1]) : s0;
-#pragma unroll
-			for(int e = 0; e < VEC; ++e) w[e] *= rem + e < P.fps ? s0 : s1;
-		} else {
-#pragma unroll
-			for(int e = 0; e < VEC; ++e) w[e] *= static_cast<Acc>(P.scales[(j0 + e) / P.fps]);
-		}
-	}
+	if constexpr(SC) scale_vec<Acc, VEC>(P, o, i, w);
 	Acc* dst = static_cast<Acc*>(C) + o * P.sCo + i;
 	if constexpr(BYTES == 16) {
 		u32x4 q;
@@ -372,6 +381,68 @@ __device__ __forceinline__ void col_unit(const Args& P, int64_t og, int64_t itil
 template<typename Acc, int VEC, int U, bool SC>
 __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, int64_t og, int64_t itile, int os,
                                           int it, int ks, int t, bool active, Acc (&v)[U][VEC], Acc* red);
+
+// Instantiations that carry the wave-transposed store (its staging raises the register count): BF16
+// lane vectors (32 bytes) with 2 or 4 slab units, FP8 lane vectors (64 bytes) with 4 units.  The
+// one-unit kernels and the two-unit FP8 kernel would lose a wave per SIMD, which costs more than
+// the stores gain.
+__host__ __device__ constexpr bool tstore_compiled(int lane_bytes, int units)
+{
+	return (lane_bytes == 32 && units >= 2) || (lane_bytes == 64 && units == 4);
+}
+
+// Column outputs of one wave whose 64 lanes lie along i (IT a multiple of 64, no reduction) and
+// whose lane vectors span G = VEC*sizeof(Acc)/16 > 1 16-byte pieces (BF16: 2, FP8: 4).  Stored
+// directly, store instruction c would write piece c of every lane, a quarter or half of each
+// 128-byte line, and lines are only completed by later instructions.  Instead the wave stages its
+// run of 64*VEC outputs in its own LDS slice (64*VEC values) and store c writes pieces
+// c*64 .. c*64+63 of the run, one per lane and contiguous, so each instruction covers whole lines
+// and can be non-temporal.  Only the wave's own lanes touch the slice, so wavefront-scope fences
+// order the accesses; every lane of the wave executes this (uniform control flow).
+template<typename Acc, int VEC, int U, bool SC>
+__device__ __forceinline__ void col_store_transposed(void* __restrict__ C, const Args& P, int64_t og, int64_t itile,
+                                                     int os, int it, bool active, const Acc (&v)[U][VEC], Acc* lds)
+{
+	static_assert(sizeof(Acc) == 4, "4-byte outputs");
+	constexpr int G = VEC * static_cast<int>(sizeof(Acc)) / 16;
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	Acc* const slice = lds + (static_cast<int>(threadIdx.x) >> 6) * (64 * VEC);
+#pragma unroll
+	for(int u = 0; u < U; ++u) {
+		int64_t oo, ii;
+		col_unit(P, og, itile, os, it, u, U, VEC, oo, ii);
+		bool const row_ok = active && oo < P.O;
+		Acc w[VEC];
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) w[e] = v[u][e];
+		if constexpr(SC)
+			if(row_ok && ii < P.I) scale_vec<Acc, VEC>(P, oo, ii, w);
+		// the previous unit's reads of the slice are done before it is written again
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+#pragma unroll
+		for(int g = 0; g < G; ++g) {
+			u32x4 q;
+			q.x = __float_as_uint(w[4 * g]);     q.y = __float_as_uint(w[4 * g + 1]);
+			q.z = __float_as_uint(w[4 * g + 2]); q.w = __float_as_uint(w[4 * g + 3]);
+			reinterpret_cast<u32x4*>(slice)[lane * G + g] = q;
+		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		int64_t const i0 = ii - static_cast<int64_t>(lane) * VEC;      // first column of the wave's run
+#pragma unroll
+		for(int c = 0; c < G; ++c) {
+			int const piece = c * 64 + lane;
+			u32x4 const q = reinterpret_cast<const u32x4*>(slice)[piece];
+			// the VEC columns of the piece's source lane are inside the row (VEC divides I)
+			if(row_ok && i0 + static_cast<int64_t>(piece / G) * VEC < P.I) {
+				Acc* const dst = static_cast<Acc*>(C) + oo * P.sCo + i0 + static_cast<int64_t>(piece) * 4;
+				__builtin_nontemporal_store(q, TTVX_GLOBAL_W(u32x4, dst));
+			}
+		}
+	}
+}
 
 template<typename S, int VEC, int U, bool KAHAN>
 __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
@@ -508,6 +579,12 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 		return;
 	}
 
+	if constexpr(tstore_compiled(VEC * static_cast<int>(sizeof(Acc)), U)) {
+		if(P.tstore) {
+			col_store_transposed<Acc, VEC, U, SC>(C, P, og, itile, os, it, active, v, red);
+			return;
+		}
+	}
 	if(active && ks == 0) {
 #pragma unroll
 		for(int u = 0; u < U; ++u) {
@@ -524,7 +601,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 template<typename S, int VEC, int U>
 __device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
                                                     void* __restrict__ C, const Args& P, int64_t ib, int64_t ie,
-                                                    int64_t is, int64_t kb, int64_t ke)
+                                                    int64_t is, int64_t kb, int64_t ke, typename Traits<S>::acc* red)
 {
 	using Acc = typename Traits<S>::acc;
 	int const t  = static_cast<int>(threadIdx.x);
@@ -561,7 +638,7 @@ __device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, con
 #pragma unroll
 			for(int e = 0; e < VEC; ++e) v[u][e] = x[e] * bk;
 		}
-		col_store<Acc, VEC, U, kScaled<S>>(C, P, og, itile, os, it, ks, t, active, v, nullptr);
+		col_store<Acc, VEC, U, kScaled<S>>(C, P, og, itile, os, it, ks, t, active, v, red);
 	};
 	Vec<S, VEC> ra[U], rb[U];
 	if(ib < ie) load(ib, ra);
@@ -586,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const t
 	int64_t ib, ie, is;
 	block_range(P.contig, P.items, ib, ie, is);
 	if(P.one_step) {
-		col_items_pipelined<S, VEC, U>(A, b, C, P, ib, ie, is, kb, ke);
+		col_items_pipelined<S, VEC, U>(A, b, C, P, ib, ie, is, kb, ke, red);
 		return;
 	}
 	for(int64_t item = ib; item < ie; item += is)

@@ -462,6 +462,14 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 		       (cfg.vec * ab) % chunk != 0))
 			chunk >>= 1;
 		cfg.args.cvec = chunk > static_cast<int>(ab) ? chunk : 0;
+		// wave-transposed stores: no reduction, a whole wave along i, lane vectors wider than 16
+		// bytes in an instantiation that carries them, 16-byte aligned C, no split partials
+		cfg.args.tstore = (cfg.args.red == 0 && cfg.args.IT % 64 == 0 && cfg.args.cvec >= 16 &&
+		                   tstore_compiled(static_cast<int>(cfg.vec * ab), cfg.U) && cfg.ksplit == 1 &&
+		                   !cfg.args.part64) ? 1 : 0;
+		// their LDS staging: one run of 64*VEC outputs per wave
+		if(cfg.args.tstore)
+			cfg.shmem = std::max<std::size_t>(cfg.shmem, static_cast<std::size_t>(kBlock) * cfg.vec * static_cast<std::size_t>(ab));
 	}
 
 	if(cfg.kind == KIND_STRIDED) {
