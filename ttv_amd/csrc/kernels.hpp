@@ -13,7 +13,8 @@
 //   k_col  (I > 1; reference gemv_col / gemv_col_blas, matrix_times_vector.h:108-221): IT lanes
 //           along i (16-byte vectors, coalesced), KS lane groups along k, OS output slabs per
 //           item; partial sums over k meet by recursive halving (one wave) or in LDS (several
-//           waves).
+//           waves).  Outputs leave with non-temporal stores where one wave instruction writes
+//           whole 128-byte lines (BF16/FP8 lane vectors are first transposed through LDS).
 //   k_strided  general strides that do not fold to three dimensions: one thread per output.
 //   k_finalize sums the split-n_q partials (FP32 or FP64) and applies FP8 scales.
 //
