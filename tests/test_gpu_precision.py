@@ -161,19 +161,23 @@ def test_kahan_long_contraction(ttvx, device):
     assert e_kahan <= e_plain
 
 
-@pytest.mark.parametrize("rows", ["narrow", "wide"])
+@pytest.mark.parametrize("rows", ["narrow", "wide", "short"])
 @pytest.mark.parametrize("fmt", ["f32", "bf16", "fp8"])
 def test_output_alignment_padding_and_scale_groups(ttvx, device, fmt, rows):
     """Outputs written by vector stores: C at every 4-byte offset off the 16-byte grid, C with
     padded rows, every kernel variant that can run, and FP8 scale groups smaller and larger than
     a lane's vector (fibers_per_scale 1, 3, 16, 1000): each output equals the FP64 oracle on the
     values the kernel reads.  "wide": 300 vectors per output row, so whole waves lie along i
-    (the wave-transposed stores of BF16/FP8) and the last tile of 256 lanes is partly past the row."""
+    (the wave-transposed stores of BF16/FP8) and the last tile of 256 lanes is partly past the row.
+    "short": 16 columns and 16 k steps, so lane groups along k meet by shuffles and a lane may
+    end with whole lane vectors (the vector stores of that path)."""
     import torch
     rng = np.random.default_rng(44)
     vec = {"f32": 4, "bf16": 8, "fp8": 16}[fmt]
     if rows == "narrow":
         na, pia, q = [64, 8, 6], [1, 2, 3], 2      # I = 64 columns (16-byte vectors), K = 8, O = 6
+    elif rows == "short":
+        na, pia, q = [16, 16, 40], [1, 2, 3], 2
     else:
         na, pia, q = [300 * vec, 4, 5], [1, 2, 3], 2
     ni, no = na[0], na[2]

@@ -541,6 +541,23 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 		int base, n;
 		bool const st = halving_sum<Acc, U * VEC>(w, t & 63, P.IT, P.KS, base, n);
 		if(active && st) {
+			if constexpr(VEC * sizeof(Acc) == 16) {
+				// the lane holds whole 16-byte lane vectors (KS <= U): one vector store each, so
+				// the lanes of one instruction write runs of slab rows (non-temporal, whole lines)
+				if(n >= VEC) {
+#pragma unroll
+					for(int g = 0; g < U; ++g)
+						if(g * VEC < n) {
+							int64_t oo, ii;
+							col_unit(P, og, itile, os, it, base / VEC + g, U, VEC, oo, ii);
+							Acc x[VEC];
+#pragma unroll
+							for(int e = 0; e < VEC; ++e) x[e] = w[g * VEC + e];
+							if(oo < P.O && ii < P.I) store_vec<Acc, VEC, SC>(C, P, oo, ii, x);
+						}
+					return;
+				}
+			}
 #pragma unroll
 			for(int j = 0; j < U * VEC; ++j)
 				if(j < n) {
