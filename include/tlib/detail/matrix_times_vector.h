@@ -13,7 +13,6 @@
 
 #include <stdexcept>
 #include <type_traits>
-#include <vector>
 
 #include "../../ttvx.h"
 
@@ -22,34 +21,22 @@ namespace tlib::ttv::detail
 
 namespace mtv_impl
 {
-// C[j] = sum_k A(j, k) b[k] for the 2-D view A(j,k) = a[j*wj + k*wk], j < M, k < N
+// c[j] (+)= sum_k A(j, k) b[k] for the 2-D view A(j,k) = a[j*wj + k*wk], j < M, k < N; the
+// accumulation runs on the GPU, so c may be host or device memory in both forms
 template<class value_t, class size_t>
-inline void run(value_t const* a, value_t const* b, value_t* c, size_t M, size_t N, size_t wj, size_t wk)
+inline void run(value_t const* a, value_t const* b, value_t* c, size_t M, size_t N, size_t wj, size_t wk,
+                bool accumulate = false)
 {
 	static_assert(std::is_same_v<value_t, float> || std::is_same_v<value_t, double>,
 	              "tlib::ttv::detail gemv/dot on MI355X support float and double.");
-	std::size_t const na[2] = {static_cast<std::size_t>(M), static_cast<std::size_t>(N)};
-	std::size_t const wa[2] = {static_cast<std::size_t>(wj), static_cast<std::size_t>(wk)};
-	// layout: the mode with the smaller stride first (ties keep mode 1 first)
-	std::size_t const pia[2] = {wj <= wk ? std::size_t{1} : std::size_t{2}, wj <= wk ? std::size_t{2} : std::size_t{1}};
-	std::size_t const nb[1] = {static_cast<std::size_t>(N)};
-	std::size_t const nc[1] = {static_cast<std::size_t>(M)};
-	std::size_t const wc[1] = {1};
-	std::size_t const pic[1] = {1};
+	auto const m = static_cast<std::size_t>(M), n = static_cast<std::size_t>(N);
+	auto const sj = static_cast<std::size_t>(wj), sk = static_cast<std::size_t>(wk);
 	int status;
 	if constexpr(std::is_same_v<value_t, float>)
-		status = ttvx_ttv_f32(TTVX_EXEC_PAR_BLAS, TTVX_SLICE_SLICE, TTVX_FUSE_NONE, 2, 2, a, na, wa, pia, b, nb, c, nc, wc, pic);
+		status = ttvx_mtv_f32(a, b, c, m, n, sj, sk, accumulate ? 1 : 0);
 	else
-		status = ttvx_ttv_f64(TTVX_EXEC_PAR_BLAS, TTVX_SLICE_SLICE, TTVX_FUSE_NONE, 2, 2, a, na, wa, pia, b, nb, c, nc, wc, pic);
+		status = ttvx_mtv_f64(a, b, c, m, n, sj, sk, accumulate ? 1 : 0);
 	if(status != TTVX_OK) throw std::runtime_error(ttvx_last_error());
-}
-
-template<class value_t, class size_t>
-inline void run_accumulate(value_t const* a, value_t const* b, value_t* c, size_t M, size_t N, size_t wj, size_t wk)
-{
-	std::vector<value_t> tmp(static_cast<std::size_t>(M));
-	run(a, b, tmp.data(), M, N, wj, wk);
-	for(std::size_t j = 0; j < static_cast<std::size_t>(M); ++j) c[j] += tmp[j];
 }
 } // namespace mtv_impl
 
@@ -74,13 +61,13 @@ inline void gemv_row_blas(value_t const* const a, value_t const* const b, value_
 template<class value_t, class size_t>
 inline void gemv_col(value_t const* const a, value_t const* const b, value_t* const c, size_t const M, size_t const N, size_t const lda)
 {
-	mtv_impl::run_accumulate(a, b, c, M, N, size_t{1}, lda);
+	mtv_impl::run(a, b, c, M, N, size_t{1}, lda, true);
 }
 
 template<class value_t, class size_t>
 inline void gemv_col_parallel(value_t const* const a, value_t const* const b, value_t* const c, size_t const M, size_t const N, size_t const lda)
 {
-	mtv_impl::run_accumulate(a, b, c, M, N, size_t{1}, lda);
+	mtv_impl::run(a, b, c, M, N, size_t{1}, lda, true);
 }
 
 template<class value_t, class size_t>

@@ -12,8 +12,6 @@
 #pragma once
 
 #include <cstddef>
-#include <functional>
-#include <numeric>
 #include <stdexcept>
 #include <type_traits>
 #include <utility>
@@ -32,31 +30,15 @@ namespace tlib::ttv
 template<class value_t>
 class device_tensor;
 
-template<class _value_t>
-struct device_tensor_view
-{
-public:
-	using value_t  = _value_t;
-	using tensor_t = device_tensor<value_t>;
-	device_tensor_view() = delete;
-	device_tensor_view(device_tensor_view const&) = delete;
-	device_tensor_view& operator=(device_tensor_view const&) = delete;
-	inline tensor_t const& get_tensor() const { return _tensor; }
-	inline std::size_t contraction_mode() const { return _q; }
-private:
-	friend tensor_t;
-	device_tensor_view(tensor_t const& t, std::size_t q) : _tensor(t), _q(q) {}
-	tensor_t const& _tensor;
-	std::size_t _q;
-};
+// What A(q) returns for a device tensor.
+template<class value_t>
+using device_tensor_view = detail::mode_view<device_tensor<value_t>>;
 
 template<class value_t>
 class device_tensor
 {
 	static_assert(std::is_same_v<value_t, float> || std::is_same_v<value_t, double>,
 	              "tlib::ttv::device_tensor supports float and double elements.");
-	using shape_t  = std::vector<std::size_t>;
-	using layout_t = std::vector<std::size_t>;
 
 	static void check(int status)
 	{
@@ -64,93 +46,90 @@ class device_tensor
 	}
 
 public:
+	using value_type = value_t;
+	using shape_t    = std::vector<std::size_t>;
+	using layout_t   = std::vector<std::size_t>;
+
 	device_tensor() = delete;
 	device_tensor(device_tensor const&) = delete;
 	device_tensor& operator=(device_tensor const&) = delete;
 
 	// uninitialised device storage of the given shape and layout
-	device_tensor(shape_t const& n, layout_t const& pi) : _n(n), _pi(pi)
-	{
-		if(n.size() != pi.size())
-			throw std::runtime_error("Error in tlib::tensor: shape vector and layout vector must have the same length.");
-		if(!detail::is_valid_shape(n.begin(), n.end()))
-			throw std::runtime_error("Error in tlib::tensor: shape vector of tensor is not valid.");
-		if(!detail::is_valid_layout(pi.begin(), pi.end()))
-			throw std::runtime_error("Error in tlib::tensor: layout vector of tensor is not valid.");
-		_size = std::accumulate(n.begin(), n.end(), std::size_t{1}, std::multiplies<>());
-		void* p = nullptr;
-		check(ttvx_device_alloc(_size * sizeof(value_t), &p));
-		_data = static_cast<value_t*>(p);
-	}
-
-	explicit device_tensor(shape_t const& n) : device_tensor(n, detail::generate_k_order_layout(n.size(), std::size_t{1})) {}
+	device_tensor(shape_t const& n, layout_t const& pi) : m_meta(n, pi) { allocate(); }
+	explicit device_tensor(shape_t const& n) : m_meta(n) { allocate(); }
 
 	// upload of a host tensor (same shape and layout)
-	explicit device_tensor(tensor<value_t> const& host) : device_tensor(host.shape(), host.layout())
+	explicit device_tensor(tensor<value_t> const& host) : m_meta(host.shape(), host.layout())
 	{
+		allocate();
 		upload(host);
 	}
 
-	device_tensor(device_tensor&& o) noexcept
-		: _n(std::move(o._n)), _pi(std::move(o._pi)), _size(o._size), _data(std::exchange(o._data, nullptr))
-	{
-	}
+	device_tensor(device_tensor&& o) noexcept : m_meta(std::move(o.m_meta)), m_dev(std::exchange(o.m_dev, nullptr)) {}
 
 	device_tensor& operator=(device_tensor&& o) noexcept
 	{
 		if(this != &o) {
-			(void)ttvx_device_free(_data);
-			_n = std::move(o._n);
-			_pi = std::move(o._pi);
-			_size = o._size;
-			_data = std::exchange(o._data, nullptr);
+			(void)ttvx_device_free(m_dev);
+			m_meta = std::move(o.m_meta);
+			m_dev = std::exchange(o.m_dev, nullptr);
 		}
 		return *this;
 	}
 
-	~device_tensor() { (void)ttvx_device_free(_data); }
+	~device_tensor() { (void)ttvx_device_free(m_dev); }
 
 	void upload(tensor<value_t> const& host)
 	{
-		if(host.shape() != _n || host.layout() != _pi)
-			throw std::runtime_error("Error in tlib::device_tensor: host tensor has another shape or layout.");
-		check(ttvx_copy(_data, host.data().data(), _size * sizeof(value_t)));
+		require_same_form(host);
+		check(ttvx_copy(m_dev, host.data().data(), bytes()));
 	}
 
 	void download(tensor<value_t>& host) const
 	{
-		if(host.shape() != _n || host.layout() != _pi)
-			throw std::runtime_error("Error in tlib::device_tensor: host tensor has another shape or layout.");
-		check(ttvx_copy(host.data().data(), _data, _size * sizeof(value_t)));
+		require_same_form(host);
+		check(ttvx_copy(host.data().data(), m_dev, bytes()));
 	}
 
 	tensor<value_t> to_host() const
 	{
-		tensor<value_t> host(_n, _pi);
+		tensor<value_t> host(shape(), layout());
 		download(host);
 		return host;
 	}
 
 	device_tensor_view<value_t> operator()(std::size_t contraction_mode) const
 	{
-		if(1ul > contraction_mode || contraction_mode > this->order())
-			throw std::runtime_error("Error in tlib::tensor: specified contraction mode should be greater than one and equal to or less than the order.");
+		m_meta.require_mode(contraction_mode);
 		return device_tensor_view<value_t>(*this, contraction_mode);
 	}
 
-	value_t const* data() const { return _data; }
-	value_t* data() { return _data; }
-	auto const& shape() const { return _n; }
-	auto const& layout() const { return _pi; }
-	auto strides() const { return detail::generate_strides(_n, _pi); }
-	auto order() const { return _n.size(); }
-	auto size() const { return _size; }
+	value_t const* data() const { return m_dev; }
+	value_t* data() { return m_dev; }
+	shape_t const& shape() const { return m_meta.extents(); }
+	layout_t const& layout() const { return m_meta.layout(); }
+	std::vector<std::size_t> strides() const { return m_meta.strides(); }
+	std::size_t order() const { return m_meta.order(); }
+	std::size_t size() const { return m_meta.elements(); }
 
 private:
-	shape_t _n;
-	layout_t _pi;
-	std::size_t _size = 0;
-	value_t* _data = nullptr;
+	std::size_t bytes() const { return m_meta.elements() * sizeof(value_t); }
+
+	void allocate()
+	{
+		void* p = nullptr;
+		check(ttvx_device_alloc(bytes(), &p));
+		m_dev = static_cast<value_t*>(p);
+	}
+
+	void require_same_form(tensor<value_t> const& host) const
+	{
+		if(host.shape() != shape() || host.layout() != layout())
+			throw std::runtime_error("Error in tlib::device_tensor: host tensor has another shape or layout.");
+	}
+
+	detail::tensor_meta m_meta;
+	value_t* m_dev = nullptr;
 };
 
 // C = A x_q b with A, b and C in device memory (reference include/tlib/ttv.h:99-114); the policy

@@ -170,6 +170,62 @@ static void mtv_tests()
 	double d = 0;
 	dot<double, std::size_t>(a.data(), a.data(), &d, 4);
 	CHECK(d == 30, "dot");
+
+	// the same kernels on device-resident operands (the accumulating forms add on the GPU)
+	{
+		void *da = nullptr, *db = nullptr, *dc = nullptr;
+		std::vector<double> ha = {1, 2, 3, 4}, hb = {1, 1}, hc = {10, 20}, back(2, 0.0);
+		CHECK(ttvx_device_alloc(4 * sizeof(double), &da) == TTVX_OK && ttvx_device_alloc(2 * sizeof(double), &db) == TTVX_OK &&
+		          ttvx_device_alloc(2 * sizeof(double), &dc) == TTVX_OK, "device alloc: %s", ttvx_last_error());
+		ttvx_copy(da, ha.data(), 4 * sizeof(double));
+		ttvx_copy(db, hb.data(), 2 * sizeof(double));
+		ttvx_copy(dc, hc.data(), 2 * sizeof(double));
+		auto* A = static_cast<double*>(da);
+		auto* B = static_cast<double*>(db);
+		auto* C = static_cast<double*>(dc);
+		auto fetch = [&]() { ttvx_copy(back.data(), dc, 2 * sizeof(double)); };
+		gemv_col<double, std::size_t>(A, B, C, 2, 2, 2);
+		fetch();
+		CHECK(back[0] == 14 && back[1] == 26, "device gemv_col accumulates: %g %g", back[0], back[1]);
+		gemv_col_parallel<double, std::size_t>(A, B, C, 2, 2, 2);
+		fetch();
+		CHECK(back[0] == 18 && back[1] == 32, "device gemv_col_parallel accumulates: %g %g", back[0], back[1]);
+		gemv_col_blas<double, std::size_t>(A, B, C, 2, 2, 2);
+		fetch();
+		CHECK(back[0] == 4 && back[1] == 6, "device gemv_col_blas overwrites: %g %g", back[0], back[1]);
+		gemv_row<double, std::size_t>(A, B, C, 2, 2, 2);
+		fetch();
+		CHECK(back[0] == 3 && back[1] == 7, "device gemv_row: %g %g", back[0], back[1]);
+		dot<double, std::size_t>(A, A, C, 4);
+		fetch();
+		CHECK(back[0] == 30, "device dot: %g", back[0]);
+		ttvx_device_free(da);
+		ttvx_device_free(db);
+		ttvx_device_free(dc);
+	}
+	// the column-major closed forms again with every operand on the device, accumulating onto c = 1
+	for(std::size_t e = 0; e < 6; ++e) {
+		std::size_t const m = std::size_t{2} << e, n = std::size_t{4} << e;
+		std::vector<float> ha(m * n), hb(n, 1.f), hc(m, 1.f), back(m);
+		for(std::size_t j = 0; j < n; ++j)
+			for(std::size_t i = 0; i < m; ++i) ha[i + j * m] = static_cast<float>(i * n + j + 1);
+		void *da = nullptr, *db = nullptr, *dc = nullptr;
+		ttvx_device_alloc(ha.size() * sizeof(float), &da);
+		ttvx_device_alloc(n * sizeof(float), &db);
+		ttvx_device_alloc(m * sizeof(float), &dc);
+		ttvx_copy(da, ha.data(), ha.size() * sizeof(float));
+		ttvx_copy(db, hb.data(), n * sizeof(float));
+		ttvx_copy(dc, hc.data(), m * sizeof(float));
+		gemv_col<float, std::size_t>(static_cast<float*>(da), static_cast<float*>(db), static_cast<float*>(dc), m, n, m);
+		ttvx_copy(back.data(), dc, m * sizeof(float));
+		auto fn = [n](std::size_t i) { return (i * n * (i * n + 1)) / 2; };
+		bool ok = true;
+		for(std::size_t i = 1; i <= m; ++i) ok = ok && back[i - 1] == static_cast<float>(fn(i) - fn(i - 1) + 1);
+		CHECK(ok, "device gemv_col %zux%zu onto ones", m, n);
+		ttvx_device_free(da);
+		ttvx_device_free(db);
+		ttvx_device_free(dc);
+	}
 }
 
 // Device-resident tensors (include/tlib/device_tensor.h): the interface1 example with the data in

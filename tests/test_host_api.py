@@ -24,3 +24,32 @@ def test_ttv_rejects_mixed_dtypes():
     import ttv_amd
     with pytest.raises(ttv_amd.TTVError):
         ttv_amd.ttv(1, np.zeros((2, 3), dtype=np.float32), np.zeros(2, dtype=np.float64))
+
+
+def test_ttv_rejects_wrong_outputs_and_vectors():
+    """A given `out` must have C's shape and A's element type; b must be a vector."""
+    import ttv_amd
+    A = np.zeros((2, 3, 4), dtype=np.float32)
+    b = np.zeros(3, dtype=np.float32)
+    with pytest.raises(ttv_amd.TTVError, match="out must be float32"):
+        ttv_amd.ttv(2, A, b, out=np.zeros((2, 4), dtype=np.float64))
+    with pytest.raises(ttv_amd.TTVError, match="out must have shape"):
+        ttv_amd.ttv(2, A, b, out=np.zeros((4, 2), dtype=np.float32))
+    with pytest.raises(ttv_amd.TTVError, match="1-D vector"):
+        ttv_amd.ttv(2, A, np.zeros((3, 1), dtype=np.float32))
+
+
+def test_reduced_precision_entry_points_check_operands():
+    """ttv_bf16 / ttv_fp8 take GPU tensors of the storage type; anything else raises before a
+    library call."""
+    import torch
+
+    import ttv_amd
+    a16 = torch.zeros((2, 3, 4), dtype=torch.uint16)
+    b = torch.zeros(3)
+    with pytest.raises(ttv_amd.TTVError, match="uint16 GPU tensor"):
+        ttv_amd.ttv_bf16(2, a16, b)
+    with pytest.raises(ttv_amd.TTVError, match="uint16 GPU tensor"):
+        ttv_amd.ttv_bf16(2, torch.zeros((2, 3, 4)), b)
+    with pytest.raises(ttv_amd.TTVError, match="uint8 GPU tensor"):
+        ttv_amd.ttv_fp8(2, a16, torch.ones(1), 8, b)

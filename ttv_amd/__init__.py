@@ -56,10 +56,37 @@ def _stream_ptr():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _vector(b, name="b"):
+    """b as a 1-D vector with unit stride: a contiguous copy when it is strided."""
+    if _is_torch(b):
+        if b.dim() != 1:
+            raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd: %s must be a 1-D vector" % name)
+        return b if b.stride(0) == 1 else b.contiguous()
+    arr = np.asarray(b)
+    if arr.ndim != 1:
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd: %s must be a 1-D vector" % name)
+    return np.ascontiguousarray(arr)
+
+
+def _checked_out(out, nc, dtype_name, need_device):
+    """Strides of a caller-supplied output after checking its element type, shape and placement."""
+    shape, strides, _, dt, dev = _describe(out)
+    if dt != dtype_name:
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd: out must be %s, got %s" % (dtype_name, dt))
+    if list(shape) != list(nc):
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd: out must have shape %s (A's shape without mode q), got %s"
+                       % (list(nc), list(shape)))
+    if need_device and not dev:
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd: out must be a GPU tensor")
+    return strides
+
+
 def ttv(q, A, b, *, flags=0, out=None):
     """C = A x_q b (q one-based).  C has A's shape without mode q and the reference's output
-    layout (layout.h:143-207: A's layout with q removed) with dense strides."""
+    layout (layout.h:143-207: A's layout with q removed) with dense strides.  b is a 1-D vector of
+    A's element type (a strided b is copied); a given `out` must have C's shape and A's type."""
     na, wa, pa, dt, dev = _describe(A)
+    b = _vector(b)
     nb, _, pb, dtb, devb = _describe(b)
     p = len(na)
     if dt not in ("float32", "float64") or dtb != dt:
@@ -84,7 +111,9 @@ def ttv(q, A, b, *, flags=0, out=None):
             span = 1 + sum((n - 1) * w for n, w in zip(nc, wc))
             buf = np.empty(span, dtype=npdt)
             out = np.lib.stride_tricks.as_strided(buf, shape=nc, strides=[w * buf.itemsize for w in wc])
-    _, wc, pc, dtc, devc = _describe(out)
+    else:
+        _checked_out(out, nc, dt, False)
+    _, wc, pc, _, devc = _describe(out)
     args = (q, p, ctypes.c_void_p(pa), sizes(na), sizes(wa), sizes(pia), ctypes.c_void_p(pb), sizes(nb),
             ctypes.c_void_p(pc), sizes(nc), sizes(wc), sizes(pic))
     if dev and devb and devc:
@@ -196,21 +225,35 @@ def quantize_fp8(q, A, fibers_per_scale=None):
     return codes.as_strided(na, wa), scales, fps
 
 
-def _out_for(q, na, pia, device):
+def _quantised_operands(q, Aq, storage, b, out, what):
+    """Checked (na, wa, pia, b, C, nc, wc, pic) of a reduced-precision product: Aq a GPU tensor of
+    the storage type, b a float32 GPU vector (copied when strided), C float32 of C's shape."""
     import torch
+    if not (_is_torch(Aq) and Aq.is_cuda and Aq.dtype == storage):
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd.%s: A must be a %s GPU tensor" % (what, str(storage).replace("torch.", "")))
+    na, wa = list(Aq.shape), list(Aq.stride())
+    if not (1 <= q <= len(na)):
+        raise TTVError(_lib.ERR_ARGUMENT, "Error in tlib::tensor_times_vector: contraction mode should be greater zero or less than or equal to p.")
+    pia = layout_from_strides(na, wa)
+    if not _is_torch(b) or not b.is_cuda or b.dtype != torch.float32:
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd.%s: b must be a float32 GPU vector" % what)
+    b = _vector(b)
     nc = generate_output_shape(na, q)
     pic = generate_output_layout(pia, q)
-    wc = generate_strides(nc, pic)
-    return torch.empty_strided(nc, wc, dtype=torch.float32, device=device), nc, wc, pic
+    if out is None:
+        wc = generate_strides(nc, pic)
+        out = torch.empty_strided(nc, wc, dtype=torch.float32, device=Aq.device)
+    else:
+        if not _is_torch(out):
+            raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd.%s: out must be a GPU tensor" % what)
+        wc = _checked_out(out, nc, "float32", True)
+    return na, wa, pia, b, out, nc, wc, pic
 
 
 def ttv_bf16(q, Aq, b, *, flags=0, out=None):
     """C = A x_q b for bfloat16 A (torch.uint16 bits), float32 b and C."""
-    na, wa = list(Aq.shape), list(Aq.stride())
-    pia = layout_from_strides(na, wa)
-    c, nc, wc, pic = _out_for(q, na, pia, Aq.device) if out is None else (out, generate_output_shape(na, q),
-                                                                           list(out.stride()),
-                                                                           generate_output_layout(pia, q))
+    import torch
+    na, wa, pia, b, c, nc, wc, pic = _quantised_operands(q, Aq, torch.uint16, b, out, "ttv_bf16")
     check(lib().ttvx_ttv_bf16_async(q, len(na), ctypes.c_void_p(Aq.data_ptr()), sizes(na), sizes(wa), sizes(pia),
                                     ctypes.c_void_p(b.data_ptr()), sizes([b.shape[0]]), ctypes.c_void_p(c.data_ptr()),
                                     sizes(nc), sizes(wc), sizes(pic), flags, _stream_ptr()))
@@ -219,11 +262,11 @@ def ttv_bf16(q, Aq, b, *, flags=0, out=None):
 
 def ttv_fp8(q, codes, scales, fibers_per_scale, b, *, flags=0, out=None):
     """C = A x_q b for e4m3 A (codes + scales from quantize_fp8), float32 b and C."""
-    na, wa = list(codes.shape), list(codes.stride())
-    pia = layout_from_strides(na, wa)
-    c, nc, wc, pic = _out_for(q, na, pia, codes.device) if out is None else (out, generate_output_shape(na, q),
-                                                                              list(out.stride()),
-                                                                              generate_output_layout(pia, q))
+    import torch
+    na, wa, pia, b, c, nc, wc, pic = _quantised_operands(q, codes, torch.uint8, b, out, "ttv_fp8")
+    if not (_is_torch(scales) and scales.is_cuda and scales.dtype == torch.float32 and scales.dim() == 1):
+        raise TTVError(_lib.ERR_ARGUMENT, "ttv_amd.ttv_fp8: scales must be a float32 GPU vector")
+    scales = _vector(scales, "scales")
     check(lib().ttvx_ttv_fp8_async(q, len(na), ctypes.c_void_p(codes.data_ptr()), sizes(na), sizes(wa), sizes(pia),
                                    ctypes.c_void_p(scales.data_ptr()), fibers_per_scale,
                                    ctypes.c_void_p(b.data_ptr()), sizes([b.shape[0]]), ctypes.c_void_p(c.data_ptr()),

@@ -41,8 +41,8 @@ bool is_device_pointer(const void* p)
 // cache's mutex (they block until C is back on the host anyway).
 struct StageCache {
 	std::mutex mutex;
-	void* buf[3] = {nullptr, nullptr, nullptr};
-	std::size_t cap[3] = {0, 0, 0};
+	void* buf[4] = {nullptr, nullptr, nullptr, nullptr};     // A, b, C, product before accumulation
+	std::size_t cap[4] = {0, 0, 0, 0};
 };
 StageCache g_stage[64];
 
@@ -50,6 +50,22 @@ struct Staged {
 	void* dev = nullptr;
 	bool staged = false;
 };
+
+// Device buffer of slot `slot` holding at least `bytes` (grown when too small).
+int stage_buffer(StageCache& cache, int slot, std::size_t bytes, void*& dev)
+{
+	if(cache.cap[slot] < bytes) {
+		if(cache.buf[slot] != nullptr) (void)hipFree(cache.buf[slot]);
+		cache.buf[slot] = nullptr;
+		cache.cap[slot] = 0;
+		std::size_t const want = std::max<std::size_t>(bytes, 1 << 20);
+		hipError_t e = hipMalloc(&cache.buf[slot], want);
+		if(e != hipSuccess) return hip_fail(e, "hipMalloc (staging)");
+		cache.cap[slot] = want;
+	}
+	dev = cache.buf[slot];
+	return TTVX_OK;
+}
 
 int stage_in(StageCache& cache, int slot, const void* host_or_dev, std::size_t bytes, bool copy, Staged& s,
              const char* what)
@@ -60,16 +76,8 @@ int stage_in(StageCache& cache, int slot, const void* host_or_dev, std::size_t b
 		s.dev = const_cast<void*>(host_or_dev);
 		return TTVX_OK;
 	}
-	if(cache.cap[slot] < bytes) {
-		if(cache.buf[slot] != nullptr) (void)hipFree(cache.buf[slot]);
-		cache.buf[slot] = nullptr;
-		cache.cap[slot] = 0;
-		std::size_t const want = std::max<std::size_t>(bytes, 1 << 20);
-		hipError_t e = hipMalloc(&cache.buf[slot], want);
-		if(e != hipSuccess) return hip_fail(e, "hipMalloc (staging)");
-		cache.cap[slot] = want;
-	}
-	s.dev = cache.buf[slot];
+	int const rc = stage_buffer(cache, slot, bytes, s.dev);
+	if(rc != TTVX_OK) return rc;
 	s.staged = true;
 	if(copy) {
 		hipError_t e = hipMemcpy(s.dev, host_or_dev, bytes, hipMemcpyHostToDevice);
@@ -78,11 +86,21 @@ int stage_in(StageCache& cache, int slot, const void* host_or_dev, std::size_t b
 	return TTVX_OK;
 }
 
+// c[j] += t[j], j < n
+template<typename T>
+__global__ __launch_bounds__(kBlock) void k_accumulate(T* __restrict__ c, const T* __restrict__ t, int64_t n)
+{
+	for(int64_t j = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; j < n; j += static_cast<int64_t>(gridDim.x) * kBlock)
+		c[j] += t[j];
+}
+
+// Synchronous product with host or device operands.  accumulate: C += A x_q b (C contiguous), the
+// semantics of the reference's non-BLAS gemv_col (matrix_times_vector.h:117-126).
 template<typename T>
 int ttv_sync(int code, std::size_t q, std::size_t p,
              const T* a, const std::size_t* na, const std::size_t* wa, const std::size_t* pia,
              const T* b, const std::size_t* nb,
-             T* c, const std::size_t* nc, const std::size_t* wc, const std::size_t* pic)
+             T* c, const std::size_t* nc, const std::size_t* wc, const std::size_t* pic, bool accumulate = false)
 {
 	clear_error();
 	Check ck = validate(q, p, a, na, wa, pia, b, nb, c, nc, wc, pic);
@@ -98,11 +116,26 @@ int ttv_sync(int code, std::size_t q, std::size_t p,
 	Staged sa, sb, sc;
 	int rc = stage_in(cache, 0, a, ab, true, sa, "A");
 	if(rc == TTVX_OK) rc = stage_in(cache, 1, b, bb, true, sb, "b");
-	// a strided C with gaps keeps the gap contents: copy it in before the kernel writes
-	if(rc == TTVX_OK) rc = stage_in(cache, 2, c, cb, f.c_span > f.outputs, sc, "C");
+	// a strided C with gaps keeps the gap contents, an accumulated C its values: copy it in first
+	if(rc == TTVX_OK) rc = stage_in(cache, 2, c, cb, accumulate || f.c_span > f.outputs, sc, "C");
 	if(rc != TTVX_OK) return rc;
-	rc = launch(f, dtype_of(code), sa.dev, sb.dev, sc.dev, nullptr, 1, 0u, -1, nullptr);
-	if(rc != TTVX_OK) return rc;
+	if(accumulate) {
+		if(f.c_span != f.outputs) return fail(TTVX_ERR_UNSUPPORTED, "ttvx: accumulation needs a contiguous C");
+		// the product goes to a device temporary (slot 3), then C += product
+		void* prod = nullptr;
+		rc = stage_buffer(cache, 3, cb, prod);
+		if(rc != TTVX_OK) return rc;
+		rc = launch(f, dtype_of(code), sa.dev, sb.dev, prod, nullptr, 1, 0u, -1, nullptr);
+		if(rc != TTVX_OK) return rc;
+		unsigned const grid = static_cast<unsigned>(std::min<int64_t>(4096, (f.outputs + kBlock - 1) / kBlock));
+		hipLaunchKernelGGL((k_accumulate<T>), dim3(std::max(grid, 1u)), dim3(kBlock), 0, nullptr, static_cast<T*>(sc.dev),
+		                   static_cast<const T*>(prod), f.outputs);
+		hipError_t const e = hipGetLastError();
+		if(e != hipSuccess) return hip_fail(e, "accumulate launch");
+	} else {
+		rc = launch(f, dtype_of(code), sa.dev, sb.dev, sc.dev, nullptr, 1, 0u, -1, nullptr);
+		if(rc != TTVX_OK) return rc;
+	}
 	hipError_t e;
 	if(sc.staged) e = hipMemcpy(c, sc.dev, cb, hipMemcpyDeviceToHost);
 	else         e = hipStreamSynchronize(nullptr);
@@ -139,6 +172,20 @@ int ttv_async(int code, std::size_t q, std::size_t p,
 	return launch(f, dt, a, b, c, scales, static_cast<int64_t>(fps), flags, -1, static_cast<hipStream_t>(stream));
 }
 
+// The 2-D view A(j, k) = a[j*wj + k*wk] as an order-2 tensor contracted over mode 2; its layout
+// puts the mode with the smaller stride first (mode 1 on ties).
+template<typename T>
+int mtv_sync(int code, const T* a, const T* b, T* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate)
+{
+	std::size_t const na[2] = {m, n};
+	std::size_t const wa[2] = {wj, wk};
+	std::size_t const pia[2] = {wj <= wk ? std::size_t{1} : std::size_t{2}, wj <= wk ? std::size_t{2} : std::size_t{1}};
+	std::size_t const nb[1] = {n};
+	std::size_t const nc[1] = {m};
+	std::size_t const wc[1] = {1};
+	std::size_t const pic[1] = {1};
+	return ttv_sync<T>(code, 2, 2, a, na, wa, pia, b, nb, c, nc, wc, pic, accumulate != 0);
+}
 } // namespace
 
 extern "C" {
@@ -206,6 +253,16 @@ int ttvx_ttv_f64(int ep, int sp, int fp, size_t q, size_t p,
 {
 	(void)ep; (void)sp; (void)fp;
 	return ttv_sync<double>(TTVX_F64, q, p, a, na, wa, pia, b, nb, c, nc, wc, pic);
+}
+
+int ttvx_mtv_f32(const float* a, const float* b, float* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate)
+{
+	return mtv_sync<float>(TTVX_F32, a, b, c, m, n, wj, wk, accumulate);
+}
+
+int ttvx_mtv_f64(const double* a, const double* b, double* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate)
+{
+	return mtv_sync<double>(TTVX_F64, a, b, c, m, n, wj, wk, accumulate);
 }
 
 int ttvx_ttv_f32_async(size_t q, size_t p,

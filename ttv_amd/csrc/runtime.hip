@@ -6,7 +6,9 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/ttvx.h"
@@ -438,8 +440,45 @@ void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
 }
 
 // ---- launch ----------------------------------------------------------------------------------
+namespace
+{
+// Split partial sums: one grow-only device buffer per (device, stream), reused in stream order by
+// every launch on that stream and held until the process ends (no allocation once it has grown to
+// the largest partials the stream needed).
+struct PartialBuffers {
+	std::mutex mutex;
+	std::map<hipStream_t, std::pair<void*, std::size_t>> by_stream;
+};
+PartialBuffers g_partials[64];
+} // namespace
+
+int partial_buffer(int device, hipStream_t stream, std::size_t bytes, void** out)
+{
+	PartialBuffers& pb = g_partials[device & 63];
+	std::lock_guard<std::mutex> lock(pb.mutex);
+	auto& slot = pb.by_stream[stream];
+	if(slot.second < bytes) {
+		if(slot.first != nullptr) {
+			// earlier launches on this stream may still read the old buffer
+			hipError_t e = hipStreamSynchronize(stream);
+			if(e != hipSuccess) return hip_fail(e, "hipStreamSynchronize (growing the split partials)");
+			(void)hipFree(slot.first);
+			slot = {nullptr, 0};
+		}
+		std::size_t const want = std::max<std::size_t>(bytes, std::size_t{1} << 20);
+		hipError_t e = hipMalloc(&slot.first, want);
+		if(e != hipSuccess) {
+			slot = {nullptr, 0};
+			return hip_fail(e, "hipMalloc (split partials)");
+		}
+		slot.second = want;
+	}
+	*out = slot.first;
+	return TTVX_OK;
+}
+
 int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c,
-           const float* scales, int64_t fps, unsigned flags, int variant, hipStream_t stream)
+           const float* scales, int64_t fps, unsigned flags, int variant, hipStream_t stream, const Scratch* scratch)
 {
 	DeviceInfo di;
 	if(!current_device(di)) return TTVX_ERR_DEVICE;
@@ -481,8 +520,16 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 	if(use_partial) {
 		std::size_t const esz = cfg.args.part64 ? 8 : acc_size(dt);
 		std::size_t const bytes = static_cast<std::size_t>(cfg.ksplit) * static_cast<std::size_t>(f.O * f.I) * esz;
-		hipError_t e = hipMallocAsync(&partial, bytes, stream);
-		if(e != hipSuccess) return hip_fail(e, "hipMallocAsync (split partials)");
+		if(scratch != nullptr) {
+			if(bytes > scratch->bytes)
+				return fail(TTVX_ERR_UNSUPPORTED, "ttvx: the split partial sums need " + std::to_string(bytes) +
+				                                      " bytes, more than the " + std::to_string(scratch->bytes) +
+				                                      "-byte static scratch buffer holds");
+			partial = scratch->ptr;
+		} else {
+			int const rc = partial_buffer(di.id, stream, bytes, &partial);
+			if(rc != TTVX_OK) return rc;
+		}
 		cfg.args.partial = partial;
 	} else if(cfg.kind != KIND_STRIDED) {
 		cfg.args.scales = scales;
@@ -508,10 +555,6 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 			                   cfg.ksplit, f.O, f.I, f.sCo, c, scales, fps > 0 ? fps : 1);
 		hipError_t e = hipGetLastError();
 		if(e != hipSuccess) rc = hip_fail(e, "finalize launch");
-	}
-	if(partial != nullptr) {
-		hipError_t e = hipFreeAsync(partial, stream);
-		if(e != hipSuccess && rc == TTVX_OK) rc = hip_fail(e, "hipFreeAsync (split partials)");
 	}
 	return rc;
 }

@@ -96,8 +96,19 @@ int launch_f64 (const Config& cfg, const void* a, const void* b, void* c, bool k
 int launch_bf16(const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
 int launch_fp8 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
 
-// Enqueues C = A x_q b for a folded problem with device pointers.
+// Device memory a caller provides for the split partial sums instead of the library's buffers.
+struct Scratch {
+	void* ptr = nullptr;
+	std::size_t bytes = 0;
+};
+
+// Grow-only buffer of at least `bytes` for the split partials of launches on (device, stream).
+int partial_buffer(int device, hipStream_t stream, std::size_t bytes, void** out);
+
+// Enqueues C = A x_q b for a folded problem with device pointers.  Split partial sums go to
+// `scratch` when given (an error when they do not fit), else to the stream's partials buffer.
 int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c,
-           const float* scales, int64_t fps, unsigned flags, int variant, hipStream_t stream);
+           const float* scales, int64_t fps, unsigned flags, int variant, hipStream_t stream,
+           const Scratch* scratch = nullptr);
 
 } // namespace ttvx
