@@ -96,14 +96,6 @@ int ttvx_ttv_f64(int ep, int sp, int fp, size_t q, size_t p,
                  const double* b, const size_t* nb,
                  double* c, const size_t* nc, const size_t* wc, const size_t* pic);
 
-/* 2-D slice product (matrix times vector) on the strided view A(j, k) = a[j*wj + k*wk], j < m, k < n:
- *   c[j] = sum_k A(j, k) * b[k]            accumulate == 0  (gemv_row*, gemv_col_blas, dot*)
- *   c[j] = c[j] + sum_k A(j, k) * b[k]     accumulate != 0  (gemv_col, gemv_col_parallel)
- * Replaces the slice kernels of include/tlib/detail/matrix_times_vector.h:51-295 (the non-BLAS
- * gemv_col forms add into c, :117-126).  c is contiguous; host or device pointers; synchronous. */
-int ttvx_mtv_f32(const float* a, const float* b, float* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate);
-int ttvx_mtv_f64(const double* a, const double* b, double* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate);
-
 /* Stream-ordered forms of the two above: device pointers only; flags = TTVX_FLAG_*. */
 int ttvx_ttv_f32_async(size_t q, size_t p,
                        const float* a, const size_t* na, const size_t* wa, const size_t* pia,
@@ -115,6 +107,14 @@ int ttvx_ttv_f64_async(size_t q, size_t p,
                        const double* b, const size_t* nb,
                        double* c, const size_t* nc, const size_t* wc, const size_t* pic,
                        unsigned flags, void* stream);
+
+/* 2-D slice product (matrix times vector) on the strided view A(j, k) = a[j*wj + k*wk], j < m, k < n:
+ *   c[j] = sum_k A(j, k) * b[k]            accumulate == 0  (gemv_row*, gemv_col_blas, dot*)
+ *   c[j] = c[j] + sum_k A(j, k) * b[k]     accumulate != 0  (gemv_col, gemv_col_parallel)
+ * Replaces the slice kernels of include/tlib/detail/matrix_times_vector.h:51-295 (the non-BLAS
+ * gemv_col forms add into c, :117-126).  c is contiguous; host or device pointers; synchronous. */
+int ttvx_mtv_f32(const float* a, const float* b, float* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate);
+int ttvx_mtv_f64(const double* a, const double* b, double* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate);
 
 /* ---- reduced-precision storage of A (no counterpart in the reference) ------------------ */
 /* A keeps its extents, strides and layout; only the element type changes.  Accumulation is
