@@ -116,6 +116,18 @@ int ttvx_ttv_f64_async(size_t q, size_t p,
 int ttvx_mtv_f32(const float* a, const float* b, float* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate);
 int ttvx_mtv_f64(const double* a, const double* b, double* c, size_t m, size_t n, size_t wj, size_t wk, int accumulate);
 
+/* Same as ttvx_ttv_f32_async, but allocates no device memory (no counterpart in the reference: for
+ * callers that cannot allocate, such as code under HIP graph capture or allocation-free serving
+ * loops).  The split-n_q / FP64-final partial sums go to a static 16 MiB device array of the
+ * library (one per GPU, held until the process ends) instead of the library's cached buffers;
+ * problems whose partials need more return TTVX_ERR_UNSUPPORTED.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other threads) race. */
+int ttvx_ttv_f32_async_noalloc(size_t q, size_t p,
+                               const float* a, const size_t* na, const size_t* wa, const size_t* pia,
+                               const float* b, const size_t* nb,
+                               float* c, const size_t* nc, const size_t* wc, const size_t* pic,
+                               unsigned flags, void* stream);
+
 /* ---- reduced-precision storage of A (no counterpart in the reference) ------------------ */
 /* A keeps its extents, strides and layout; only the element type changes.  Accumulation is
  * FP32 (optionally Kahan), b and C are float.

@@ -35,6 +35,8 @@ _SIGS = {
     "ttvx_ttv_f64": ([_int, _int, _int, _sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp], _int),
     "ttvx_ttv_f32_async": ([_sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp, _uint, _vp], _int),
     "ttvx_ttv_f64_async": ([_sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp, _uint, _vp], _int),
+    "ttvx_ttv_f32_async_noalloc": ([_sz, _sz, _vp, _szp, _szp, _szp, _vp, _szp, _vp, _szp, _szp, _szp, _uint, _vp],
+                                   _int),
     "ttvx_mtv_f32": ([_vp, _vp, _vp, _sz, _sz, _sz, _sz, _int], _int),
     "ttvx_mtv_f64": ([_vp, _vp, _vp, _sz, _sz, _sz, _sz, _int], _int),
     "ttvx_quantize_bf16_async": ([_sz, _vp, _vp, _vp], _int),

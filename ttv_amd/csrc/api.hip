@@ -149,7 +149,7 @@ int ttv_async(int code, std::size_t q, std::size_t p,
               const float* scales, std::size_t fps,
               const void* b, const std::size_t* nb,
               void* c, const std::size_t* nc, const std::size_t* wc, const std::size_t* pic,
-              unsigned flags, void* stream)
+              unsigned flags, void* stream, const Scratch* scratch = nullptr)
 {
 	clear_error();
 	Check ck = validate(q, p, a, na, wa, pia, b, nb, c, nc, wc, pic);
@@ -169,7 +169,7 @@ int ttv_async(int code, std::size_t q, std::size_t p,
 		rc = check_device_span(scales, ns * sizeof(float), "scales");
 		if(rc != TTVX_OK) return rc;
 	}
-	return launch(f, dt, a, b, c, scales, static_cast<int64_t>(fps), flags, -1, static_cast<hipStream_t>(stream));
+	return launch(f, dt, a, b, c, scales, static_cast<int64_t>(fps), flags, -1, static_cast<hipStream_t>(stream), scratch);
 }
 
 // The 2-D view A(j, k) = a[j*wj + k*wk] as an order-2 tensor contracted over mode 2; its layout
@@ -185,6 +185,30 @@ int mtv_sync(int code, const T* a, const T* b, T* c, size_t m, size_t n, size_t 
 	std::size_t const wc[1] = {1};
 	std::size_t const pic[1] = {1};
 	return ttv_sync<T>(code, 2, 2, a, na, wa, pia, b, nb, c, nc, wc, pic, accumulate != 0);
+}
+// Split partial sums of ttvx_ttv_f32_async_noalloc: a static array that every GPU the library runs
+// on holds until the process ends.  Host code never passes `scratch` itself (its host-side name is
+// not a device address): it uses the address hipGetSymbolAddress returns for the current device.
+constexpr std::size_t kNoAllocScratchBytes = std::size_t{16} << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[64] = {};
+
+// The current device's copy of `scratch` (looked up once per device, with that device current).
+int device_scratch(int device, Scratch& out)
+{
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	void*& ptr = g_scratch_ptr[device & 63];
+	if(ptr == nullptr) {
+		void* p = nullptr;
+		hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || p == nullptr) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress (scratch)");
+		ptr = p;
+	}
+	out.ptr = ptr;
+	out.bytes = kNoAllocScratchBytes;
+	return TTVX_OK;
 }
 } // namespace
 
@@ -272,6 +296,21 @@ int ttvx_ttv_f32_async(size_t q, size_t p,
                        unsigned flags, void* stream)
 {
 	return ttv_async<float>(TTVX_F32, q, p, a, na, wa, pia, nullptr, 0, b, nb, c, nc, wc, pic, flags, stream);
+}
+
+int ttvx_ttv_f32_async_noalloc(size_t q, size_t p,
+                               const float* a, const size_t* na, const size_t* wa, const size_t* pia,
+                               const float* b, const size_t* nb,
+                               float* c, const size_t* nc, const size_t* wc, const size_t* pic,
+                               unsigned flags, void* stream)
+{
+	clear_error();
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	Scratch sc;
+	int const rc = device_scratch(di.id, sc);
+	if(rc != TTVX_OK) return rc;
+	return ttv_async<float>(TTVX_F32, q, p, a, na, wa, pia, nullptr, 0, b, nb, c, nc, wc, pic, flags, stream, &sc);
 }
 
 int ttvx_ttv_f64_async(size_t q, size_t p,
