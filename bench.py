@@ -7,16 +7,24 @@ per step, inputs resident in HBM before timing starts.  `value` is the step's al
 (per product N*4 + (N/n_q)*4 + n_q*4 = 1,082,130,944 B) divided by the step time, in GB/s; the
 step time comes from one event pair around K back-to-back steps.
 
+Timing protocol (every leg): the launches of a timed region are enqueued behind a bounded spin
+kernel, so the host has queued them all before the GPU reaches them, and one event pair on the
+launch stream brackets the whole region; a launch's time is the region's time over its launches
+(kernel time plus the sub-microsecond gap between consecutive launches).  Consecutive launches
+read operands that together exceed the 256 MB last-level cache (1 GiB tensors by themselves;
+smaller ones rotate over copies), so every launch streams A from HBM.
+
 Also reported on the same line:
-  roofline      -- the product kernel with the largest share of the step time: algorithmic
-                   bytes per launch / its mean launch time (an event pair around each launch of a
-                   second pass of K steps, on the launch stream)
-                   against the 8 TB/s HBM peak; traffic from the committed rocprofv3 counters
+  roofline      -- the slowest of the 8 products: algorithmic bytes per launch / its mean launch
+                   time (K launches back to back) against the 8 TB/s HBM peak; traffic from the
+                   committed rocprofv3 counters
   per_case      -- GB/s and fraction of peak for each (q, layout)
-  read_ceiling  -- a pure 16-byte read stream over the same 1 GiB (the practical ceiling)
-  precision     -- BF16 and FP8-e4m3 storage of A: speedup over the FP32 kernel per case and the
-                   relative L2 error of C against the FP64 oracle on the original data (and on the
-                   dequantised data)
+  read_ceiling  -- a pure 16-byte read stream over the same 1 GiB; mixed_ceilings -- the same read
+                   stream plus a write stream of 1/128, 1/16 or 1/4 of it (the output fractions of
+                   128^4, 16^7 and the order-8 q = 4 product)
+  precision     -- BF16 and FP8-e4m3 storage of A: speedup over the FP32 kernel per case (both timed
+                   with the protocol above) and the relative L2 error of C against the FP64 oracle on
+                   the original data (and on the dequantised data)
   cpu_baseline  -- the reference's CPU path (restated in oracle/: OpenMP over slices + OpenBLAS
                    gemv, par_loop/subtensor/all) on the host's physical cores, on the same inputs
 
@@ -54,29 +62,35 @@ def spin(torch, ttv_amd, us, sink):
                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
 
-def timed_launches(torch, ttv_amd, run, reps, sink, flush=None, us_per_launch=150.0):
-    """Times `reps` launches of run() with HIP events on the current stream, all enqueued behind a
-    spin (and, with flush, each behind a write of the flush buffer that evicts the last-level cache)."""
-    run()
+def timed_cycle(torch, ttv_amd, runs, reps, sink, us_per_launch=200.0):
+    """Mean time of one launch in ms: `reps` launches back to back on the current stream, cycling
+    through `runs` (callables that each enqueue one launch, typically on their own copy of the
+    operands), behind a spin, with one event pair around all of them."""
+    for r in runs:
+        r()
     torch.cuda.synchronize()
-    spin(torch, ttv_amd, reps * (us_per_launch + (100.0 if flush is not None else 0.0)), sink)
-    evs = []
-    for _ in range(reps):
-        if flush is not None:
-            flush.zero_()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        run()
-        e1.record()
-        evs.append((e0, e1))
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    spin(torch, ttv_amd, reps * us_per_launch, sink)
+    e0.record(stream)
+    for i in range(reps):
+        runs[i % len(runs)]()
+    e1.record(stream)
     torch.cuda.synchronize()
-    return [a.elapsed_time(b) for a, b in evs]
+    return e0.elapsed_time(e1) / reps
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the dominant kernel measured with rocprofv3 counters (committed
+def copies_for(t, min_bytes=1 << 30):
+    """t and enough clones of it that cycling through them reads at least min_bytes (> the
+    last-level cache) between two reads of the same copy."""
+    n = max(1, -(-min_bytes // (t.numel() * t.element_size())))
+    return [t] + [t.clone() for _ in range(n - 1)]
+
+
+def traffic_from_profiles(name="traffic_ttv_f32_128x4.json"):
+    """HBM bytes per launch of the profiled kernels measured with rocprofv3 counters (committed
     summary), or None."""
-    path = os.path.join(REPO, "profiles", "traffic_ttv_f32_128x4.json")
+    path = os.path.join(REPO, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f)
@@ -84,14 +98,27 @@ def traffic_from_profiles():
 
 
 CASE_ORDER = [(lname, q) for lname in ("first", "last") for q in range(1, 5)]
+FP8_FIBERS_PER_SCALE = 32                        # ttv_amd.default_fibers_per_scale(128)
+FP8_SCALES = (N // 128) // FP8_FIBERS_PER_SCALE  # one float per 4096 elements
 
 
-def summarize_profiles(d, traffic_out=None):
-    """Summary text of the three rocprofv3 runs of `bench.py --profile` under directory d
+def profile_cases(kind):
+    """(label, algorithmic bytes per launch) of the products `bench.py --profile KIND` dispatches,
+    in dispatch order (each 3 times)."""
+    if kind == "f32":
+        return [("%s q=%d" % (lname, q), case_bytes(128)) for lname, q in CASE_ORDER]
+    out = []
+    for lname, q in CASE_ORDER:
+        out.append(("bf16 %s q=%d" % (lname, q), case_bytes(128, elem=2)))
+        out.append(("fp8 %s q=%d" % (lname, q), case_bytes(128, elem=1, extra=FP8_SCALES * 4)))
+    return out
+
+
+def summarize_profiles(d, traffic_out=None, kind="f32"):
+    """Summary text of the three rocprofv3 runs of `bench.py --profile KIND` under directory d
     (prof_trace: --kernel-trace --stats; prof_fetch: --pmc FETCH_SIZE; prof_write: --pmc WRITE_SIZE)
-    and, with traffic_out, the per-case HBM traffic JSON that the benchmark line reports.
-    Counters are calibrated with the plain copy kernel of the same sequence (1 GiB read, 1 GiB
-    written per launch)."""
+    and, with traffic_out, the per-case HBM traffic JSON.  Counters are calibrated with the plain
+    copy kernel of the same sequence (1 GiB read, 1 GiB written per launch)."""
     import csv
     import re
 
@@ -102,6 +129,9 @@ def summarize_profiles(d, traffic_out=None):
         with open(path) as f:
             return list(csv.DictReader(f))
 
+    def is_product(name):
+        return any(k in name for k in ("k_dot", "k_col", "k_strided"))
+
     out = []
     stats = rows(os.path.join(d, "prof_trace", "run_kernel_stats.csv"))
     out.append("Kernel statistics (run_kernel_stats.csv):")
@@ -110,12 +140,13 @@ def summarize_profiles(d, traffic_out=None):
             out.append("  %-58s calls=%3s avg_us=%8.1f min_us=%8.1f max_us=%8.1f" % (
                 short(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3,
                 float(r["MaxNs"]) / 1e3))
+
     def after_streams(rs):
         """ttvx dispatches in dispatch order; the products timed are those after the stream kernels
         (the ones before them only allocated the outputs)"""
         rs = sorted((r for r in rs if "ttvx" in r["Kernel_Name"]), key=lambda r: int(r["Dispatch_Id"]))
         last = max(i for i, r in enumerate(rs) if "stream" in r["Kernel_Name"])
-        return rs, [r for r in rs[last + 1:] if "stream" not in r["Kernel_Name"]]
+        return rs, [r for r in rs[last + 1:] if is_product(r["Kernel_Name"])]
 
     _, prod_t = after_streams(rows(os.path.join(d, "prof_trace", "run_kernel_trace.csv")))
 
@@ -132,30 +163,34 @@ def summarize_profiles(d, traffic_out=None):
     out.append("Counter calibration with the plain copy of 1 GiB (reads 1,048,576 KiB, writes 1,048,576 KiB):")
     out.append("  FETCH_SIZE read %.1f KiB -> scale %.4f ; WRITE_SIZE read %.1f KiB -> scale %.4f" % (
         fetch_copy, fscale, write_copy, wscale))
-    n_case = len(CASE_ORDER)
-    reps = len(prod_t) // n_case
-    assert reps >= 1 and len(prod_t) == reps * n_case and len(fetch) == len(prod_t) and len(write) == len(prod_t), \
-        "expected %d product dispatches per run" % (reps * n_case)
-    alg = N * 4 + (N // 128) * 4 + 128 * 4
+    plan = profile_cases(kind)
+    reps = len(prod_t) // len(plan)
+    assert reps >= 1 and len(prod_t) == reps * len(plan) and len(fetch) == len(prod_t) and len(write) == len(prod_t), \
+        "expected %d product dispatches per run" % (reps * len(plan))
     cases = []
     out.append("")
-    out.append("Per case (mean of %d launches): trace duration, HBM traffic from the scaled counters, vs %d "
-               "algorithmic bytes:" % (reps, alg))
-    for i, (lname, q) in enumerate(CASE_ORDER):
+    out.append("Per case (mean of %d launches): trace duration, HBM traffic from the scaled counters, vs the "
+               "algorithmic bytes:" % reps)
+    for i, (label, alg) in enumerate(plan):
         sl = slice(i * reps, (i + 1) * reps)
         us = sum((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in prod_t[sl]) / reps
         fb = sum(v for _, v in fetch[sl]) / reps * 1024 * fscale
         wb = sum(v for _, v in write[sl]) / reps * 1024 * wscale
         name = short(prod_t[i * reps]["Kernel_Name"])
-        cases.append(dict(layout=lname, q=q, kernel=name, avg_us=round(us, 2), fetch_bytes=round(fb),
-                          write_bytes=round(wb), hbm_bytes_per_launch=round(fb + wb),
-                          traffic_over_algorithmic=round((fb + wb) / alg, 4)))
-        out.append("  %-5s q=%d %-36s %7.1f us  fetch %.4e B  write %.4e B  (%.4f x algorithmic, %.1f GB/s)" % (
-            lname, q, name.replace("ttvx::", ""), us, fb, wb, (fb + wb) / alg, alg / (us * 1e-6) / 1e9))
+        entry = dict(label=label, kernel=name, avg_us=round(us, 2), algorithmic_bytes=alg, fetch_bytes=round(fb),
+                     write_bytes=round(wb), hbm_bytes_per_launch=round(fb + wb),
+                     traffic_over_algorithmic=round((fb + wb) / alg, 4), gbs=round(alg / (us * 1e-6) / 1e9, 1))
+        if kind == "f32":
+            entry["layout"], entry["q"] = CASE_ORDER[i]
+        cases.append(entry)
+        out.append("  %-15s %-38s %7.1f us  fetch %.4e B  write %.4e B  (%.4f x algorithmic %d B, %.1f GB/s)" % (
+            label, name.replace("ttvx::", ""), us, fb, wb, (fb + wb) / alg, alg, alg / (us * 1e-6) / 1e9))
     if traffic_out:
+        workload = ("FP32 128^4, q=1..4 x {first,last}-order" if kind == "f32" else
+                    "128^4 with A in BF16 and FP8-e4m3 (one scale per 4096 elements), q=1..4 x {first,last}-order")
         with open(traffic_out, "w") as f:
-            json.dump(dict(workload="FP32 128^4, q=1..4 x {first,last}-order", algorithmic_bytes=alg,
-                           fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), cases=cases), f, indent=1)
+            json.dump(dict(workload=workload, fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), cases=cases),
+                      f, indent=1)
     return "\n".join(out)
 
 
@@ -170,15 +205,16 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 16^7 FP32 and 256^3 FP64 cases")
     ap.add_argument("--no-order8", action="store_true", help="skip the order-8 precision configuration")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--profile", action="store_true",
+    ap.add_argument("--profile", choices=("f32", "precision"),
                     help="short fixed dispatch sequence for rocprofv3: copy and read streams, then each of the "
-                         "8 products 3 times; prints no JSON")
+                         "8 FP32 products (f32) or the 8 BF16 and 8 FP8 products (precision) 3 times; prints no JSON")
     ap.add_argument("--summarize", metavar="DIR",
                     help="print the summary of the rocprofv3 runs of --profile under DIR (no GPU needed)")
+    ap.add_argument("--kind", choices=("f32", "precision"), default="f32", help="with --summarize: the --profile kind")
     ap.add_argument("--traffic-out", metavar="JSON", help="with --summarize: write the per-case traffic here")
     args = ap.parse_args()
     if args.summarize:
-        print(summarize_profiles(args.summarize, args.traffic_out))
+        print(summarize_profiles(args.summarize, args.traffic_out, args.kind))
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are out of scope (the mode-q product "
@@ -215,17 +251,37 @@ def main():
 
     if args.profile:
         L = ttv_amd.lib()
+        # operands and outputs first: only the stream kernels and the profiled products follow
+        prep = []
+        if args.profile == "precision":
+            Abf = ttv_amd.quantize_bf16(Aflat)
+            for c in cases:
+                q, wa = c["q"], c["wa"]
+                codes, scales, fps = ttv_amd.quantize_fp8(q, Aflat.as_strided(SHAPE, wa))
+                assert fps == FP8_FIBERS_PER_SCALE and scales.numel() == FP8_SCALES
+                Aq = Abf.as_strided(SHAPE, wa)
+                prep.append((c, Aq, codes, scales, fps, ttv_amd.ttv_bf16(q, Aq, c["b"]),
+                             ttv_amd.ttv_fp8(q, codes, scales, fps, c["b"])))
         dst = torch.empty_like(Aflat)
         partial = torch.empty(256 * 32, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
         for _ in range(3):
             ttv_amd.check(L.ttvx_bench_copy(Aflat.data_ptr(), dst.data_ptr(), N * 4, stream.cuda_stream))
         for _ in range(3):
             ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
                                             stream.cuda_stream))
-        for c in cases:
-            log("profile case", c["layout"], c["q"], c["kernel"])
-            for _ in range(3):
-                run_case(c)
+        if args.profile == "f32":
+            for c in cases:
+                log("profile case", c["layout"], c["q"], c["kernel"])
+                for _ in range(3):
+                    run_case(c)
+        else:
+            for c, Aq, codes, scales, fps, Cb, C8 in prep:
+                log("profile case bf16, fp8", c["layout"], c["q"])
+                for _ in range(3):
+                    ttv_amd.ttv_bf16(c["q"], Aq, c["b"], out=Cb)
+                for _ in range(3):
+                    ttv_amd.ttv_fp8(c["q"], codes, scales, fps, c["b"], out=C8)
         torch.cuda.synchronize()
         log("profile sequence done")
         return
@@ -241,7 +297,7 @@ def main():
     t_end = torch.cuda.Event(enable_timing=True)
     sink = torch.zeros(1, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
-    spin(torch, ttv_amd, args.steps * len(cases) * 80.0, sink)     # the host enqueues the whole region
+    spin(torch, ttv_amd, args.steps * len(cases) * 200.0, sink)     # the host enqueues the whole region
     t_start.record(stream)
     for s in range(args.steps):
         for c in cases:
@@ -252,24 +308,18 @@ def main():
     step_bytes = sum(c["bytes"] for c in cases)
     value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
 
-    # per-launch region: the same K steps again with an event pair around every launch; the
-    # per-case times and the roofline of the slowest kernel come from here
-    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
-          for _ in range(args.steps)]
-    spin(torch, ttv_amd, args.steps * len(cases) * 80.0, sink)
-    for s in range(args.steps):
-        for i, c in enumerate(cases):
-            ev[s][i][0].record(stream)
+    # per-case region: each product K times back to back between one event pair (its mean launch
+    # time); all 8 regions are enqueued behind one spin
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
+    spin(torch, ttv_amd, args.steps * len(cases) * 200.0, sink)
+    for c, (e0, e1) in zip(cases, evs):
+        e0.record(stream)
+        for s in range(args.steps):
             run_case(c)
-            ev[s][i][1].record(stream)
+        e1.record(stream)
     torch.cuda.synchronize()
-    per_launch_ms = sum(ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(args.steps)
-                        for i in range(len(cases))) / args.steps
-    for i, c in enumerate(cases):
-        ts = [ev[s][i][0].elapsed_time(ev[s][i][1]) for s in range(args.steps)]
-        c["ms"] = float(np.mean(ts))
-        c["ms_min"] = float(np.min(ts))
-        c["ms_med"] = float(np.median(ts))
+    for c, (e0, e1) in zip(cases, evs):
+        c["ms"] = e0.elapsed_time(e1) / args.steps
         c["gbs"] = c["bytes"] / (c["ms"] * 1e-3) / 1e9
     per_case = [dict(layout=c["layout"], q=c["q"], kernel=c["kernel"], ms=round(c["ms"], 4),
                      gbs=round(c["gbs"], 1), frac=round(c["gbs"] * 1e9 / PEAK_BPS, 4)) for c in cases]
@@ -280,23 +330,31 @@ def main():
     achieved = dom["bytes"] / (dom["ms"] * 1e-3)
     traffic = None
     for t in (traffic_from_profiles() or {}).get("cases", []):
-        if t["layout"] == dom["layout"] and t["q"] == dom["q"]:
+        if t.get("layout") == dom["layout"] and t.get("q") == dom["q"]:
             traffic = t["hbm_bytes_per_launch"]
     roofline = dict(bound="hbm", achieved=round(achieved / 1e9, 1), peak=8000.0, unit="GB/s",
                     frac=round(achieved / PEAK_BPS, 4), traffic=traffic,
                     kernel="%s (%s-order, q=%d)" % (dom["kernel"], dom["layout"], dom["q"]),
                     algorithmic_bytes_per_launch=dom["bytes"], launch_ms=round(dom["ms"], 4))
 
-    # practical ceiling: pure read stream over the same buffer
+    # practical ceilings: a pure read stream over the same buffer, and the same read stream with a
+    # write stream of 1/128, 1/16 and 1/4 of it (the output fractions of 128^4, 16^7 and order-8 q=4)
     partial = torch.empty(256 * 32, dtype=torch.float32, device=dev)
     L = ttv_amd.lib()
-    for _ in range(3):
-        ttv_amd.check(L.ttvx_bench_read(Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(),
-                                        stream.cuda_stream))
-    rts = timed_launches(torch, ttv_amd, lambda: ttv_amd.check(L.ttvx_bench_read(
-        Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(), stream.cuda_stream)), 10, sink)
-    read_gbs = N * 4 / (np.median(rts) * 1e-3) / 1e9
+    read_ms = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.check(L.ttvx_bench_read(
+        Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(), stream.cuda_stream))], args.steps, sink)
+    read_gbs = N * 4 / (read_ms * 1e-3) / 1e9
     log("read stream ceiling: %.1f GB/s (%.3f of peak)" % (read_gbs, read_gbs * 1e9 / PEAK_BPS))
+    wbuf = torch.empty(N // 4, dtype=torch.float32, device=dev)
+    mixed = {}
+    for frac in (128, 16, 4):
+        wbytes = N * 4 // frac
+        ms = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.check(L.ttvx_bench_mix(
+            Aflat.data_ptr(), N * 4, wbuf.data_ptr(), wbytes, stream.cuda_stream))], args.steps, sink)
+        gbs = (N * 4 + wbytes) / (ms * 1e-3) / 1e9
+        mixed["write_1_%d" % frac] = {"gbs": round(gbs, 1), "frac": round(gbs * 1e9 / PEAK_BPS, 4)}
+        log("read + 1/%d write ceiling: %.1f GB/s" % (frac, gbs))
+    del wbuf
 
     out = {
         "metric": "HBM GB/s (% peak) order-4 FP32 TTV; BF16/FP8 speedup at rel err<=1e-2",
@@ -306,7 +364,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(total_ms / args.steps, 4),
-        "ms_per_step_sum_of_launch_events": round(per_launch_ms, 4),
+        "ms_per_step_sum_of_cases": round(sum(c["ms"] for c in cases), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -318,6 +376,7 @@ def main():
         "min_case_frac": round(min(pc["frac"] for pc in per_case), 4),
         "read_ceiling": {"gbs": round(read_gbs, 1), "frac": round(read_gbs * 1e9 / PEAK_BPS, 4),
                          "value_over_ceiling": round(value / read_gbs, 4)},
+        "mixed_ceilings": mixed,
         "roofline": roofline,
         "per_case": per_case,
     }
@@ -339,7 +398,7 @@ def main():
         out["batching"] = batching_leg(torch, np, ttv_amd, log)
 
     if not args.no_extra:
-        out["other_configs"] = other_configs_leg(torch, np, ttv_amd, Aflat, log)
+        out["other_configs"] = other_configs_leg(torch, np, ttv_amd, Aflat, mixed, log)
 
     if not args.no_order8:
         del Aflat
@@ -356,9 +415,9 @@ def order8_leg(torch, np, ttv_amd, log, reps=5):
     """The order-8 precision configuration: FP32 A of extents (4096,4,8,4,8,4,2,64) = 2^31 elements
     (8 GiB; the product of the extents, not the ~1.07e9 sometimes quoted for it), layout drawn from
     a seed, q in {1, 4, 8} (n_q = 4096, 4, 64); A ~ N(0,1) with 1% of entries x100, b ~ U[-1,1].
-    FP32, BF16 and FP8-e4m3 storage, FP32 accumulation: time (median of launches behind a spin,
-    cache evicted before each), speedup and its byte-bound ideal, relative L2 and max errors
-    against the FP64 oracle on the original values."""
+    FP32, BF16 and FP8-e4m3 storage, FP32 accumulation: mean launch time (launches back to back;
+    every operand is 2-8 GiB, far above the last-level cache), speedup and its byte-bound ideal,
+    relative L2 and max errors against the FP64 oracle on the original values."""
     from oracle import oracle as O
     dev = torch.device("cuda:0")
     na = [4096, 4, 8, 4, 8, 4, 2, 64]
@@ -374,7 +433,6 @@ def order8_leg(torch, np, ttv_amd, log, reps=5):
     Av = A.as_strided(na, wa)
     Abf = ttv_amd.quantize_bf16(A).as_strided(na, wa)
     sink = torch.zeros(1, dtype=torch.int32, device=dev)
-    flush = torch.empty(1 << 27, dtype=torch.float32, device=dev)
     res = {"shape": na, "layout": pia, "elements": n, "cases": []}
     for q in (1, 4, 8):
         nq = na[q - 1]
@@ -392,7 +450,7 @@ def order8_leg(torch, np, ttv_amd, log, reps=5):
                 "fp8": (lambda: ttv_amd.ttv_fp8(q, codes, scales, fps, bd, out=C8), C8, n + scales.numel() * 4)}
         case = {"q": q, "n_q": nq}
         for k, (run, C, abytes) in runs.items():
-            ms = float(np.median(timed_launches(torch, ttv_amd, run, reps, sink, flush=flush)))
+            ms = timed_cycle(torch, ttv_amd, [run], reps, sink, us_per_launch=2000.0)
             nbytes = abytes + outc * 4 + nq * 4
             got = C.as_strided([C.numel()], [1]).cpu().numpy().astype(np.float64)
             d = got - ref
@@ -429,21 +487,16 @@ def host_staged_leg(np, ttv_amd, a_host, cases, log, reps=3):
     return dict(layout=c["layout"], q=c["q"], ms=round(ms, 3), gbs_including_pcie=round(gbs, 1))
 
 
-def other_configs_leg(torch, np, ttv_amd, Aflat, log, reps=5):
+def other_configs_leg(torch, np, ttv_amd, Aflat, mixed, log, reps=10):
     """The other shapes of the reference's benchmark: FP32 16^7 (2^28 elements, every q, first- and
-    last-order layout; same data as 128^4) and FP64 256^3 first-order with q = 2.  The FP64 tensor
-    (128 MiB) fits in the 256 MB last-level cache, so a 512 MiB buffer is written before every
-    timed launch."""
+    last-order layout; same data as 128^4; each product writes 1/16 of what it reads, so its
+    ceiling is the read stream with a 1/16 write stream) and FP64 256^3 first-order with q = 2
+    (128 MiB, timed over 8 copies so that every launch reads HBM)."""
     from ttv_amd.meta import generate_strides
     dev = Aflat.device
-    flush = torch.empty(1 << 27, dtype=torch.float32, device=dev)
     sink = torch.zeros(1, dtype=torch.int32, device=dev)
-
-    def time_one(run, nbytes):
-        ms = float(np.median(timed_launches(torch, ttv_amd, run, reps, sink, flush=flush)))
-        return ms, nbytes / (ms * 1e-3) / 1e9
-
-    res = {"fp32_16e7": [], "fp64_256e3": None}
+    ceil16 = mixed["write_1_16"]["gbs"]
+    res = {"fp32_16e7": [], "fp64_256e3": None, "ceiling_16e7_gbs": ceil16}
     sh = [16] * 7
     for lname, pia in (("first", list(range(1, 8))), ("last", list(range(7, 0, -1)))):
         wa = generate_strides(sh, pia)
@@ -453,23 +506,31 @@ def other_configs_leg(torch, np, ttv_amd, Aflat, log, reps=5):
             C = ttv_amd.ttv(q, A, b)
             info = ttv_amd.plan_info(q, sh, wa, pia)
             nbytes = N * 4 + (N // 16) * 4 + 16 * 4
-            ms, gbs = time_one(lambda: ttv_amd.ttv(q, A, b, out=C), nbytes)
+            ms = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.ttv(q, A, b, out=C)], reps, sink)
+            gbs = nbytes / (ms * 1e-3) / 1e9
             res["fp32_16e7"].append(dict(layout=lname, q=q, kernel=ttv_amd.variant_names()[info["variant"]],
-                                         ms=round(ms, 4), gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4)))
-            log("  16^7 %-5s q=%d %-10s %.4f ms %7.1f GB/s" % (lname, q, res["fp32_16e7"][-1]["kernel"], ms, gbs))
+                                         ms=round(ms, 4), gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4),
+                                         of_ceiling=round(gbs / ceil16, 4)))
+            log("  16^7 %-5s q=%d %-10s %.4f ms %7.1f GB/s (%.3f of the 1/16-write ceiling)" % (
+                lname, q, res["fp32_16e7"][-1]["kernel"], ms, gbs, gbs / ceil16))
     n3 = 1 << 24
     A64 = Aflat[:n3].double().view(256, 256, 256).permute(2, 1, 0)     # first-order strides (1, 256, 65536)
+    base = Aflat[:n3].double()
+    A64s = [t.view(256, 256, 256).permute(2, 1, 0) for t in copies_for(base)]
     b64 = torch.rand(256, device=dev, dtype=torch.float64)
     C64 = ttv_amd.ttv(2, A64, b64)
     nbytes = n3 * 8 + 65536 * 8 + 256 * 8
-    ms, gbs = time_one(lambda: ttv_amd.ttv(2, A64, b64, out=C64), nbytes)
+    runs = [(lambda T=T: ttv_amd.ttv(2, T, b64, out=C64)) for T in A64s]
+    ms = timed_cycle(torch, ttv_amd, runs, reps * len(runs), sink, us_per_launch=100.0)
+    gbs = nbytes / (ms * 1e-3) / 1e9
     info = ttv_amd.plan_info(2, [256] * 3, list(A64.stride()), [1, 2, 3], ttv_amd.F64)
     res["fp64_256e3"] = dict(q=2, layout="first", kernel=ttv_amd.variant_names()[info["variant"]], ms=round(ms, 4),
-                             gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4))
+                             gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4), copies=len(A64s))
     log("  256^3 FP64 q=2 %.4f ms %7.1f GB/s" % (ms, gbs))
     fr = [c["frac"] for c in res["fp32_16e7"]]
     res["fp32_16e7_min_frac"] = min(fr)
     res["fp32_16e7_mean_frac"] = round(float(np.mean(fr)), 4)
+    res["fp32_16e7_min_of_ceiling"] = min(c["of_ceiling"] for c in res["fp32_16e7"])
     return res
 
 
@@ -495,19 +556,11 @@ def batching_leg(torch, np, ttv_amd, log, count=512, seed=4):
     seq_out = [ttv_amd.ttv(q, A, b) for q, A, b in probs]
     stream = torch.cuda.current_stream()
 
+    sink = torch.zeros(1, dtype=torch.int32, device=dev)
+
     def timed(fn, reps):
-        for _ in range(2):
-            fn()
-        torch.cuda.synchronize()
-        ts = []
-        for _ in range(reps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            fn()
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1))
-        return float(np.median(ts))
+        # every batch reads tens of GB: back-to-back repetitions are HBM-cold
+        return timed_cycle(torch, ttv_amd, [fn], reps, sink, us_per_launch=30000.0)
 
     ms_graph = timed(lambda: bt.launch(graph=True), 5)
     ms_plain = timed(lambda: bt.launch(graph=False), 5)
@@ -533,13 +586,17 @@ def batching_leg(torch, np, ttv_amd, log, count=512, seed=4):
 
 
 def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
+    """BF16 and FP8 storage of the 8 FP32 cases.  Each is timed like the FP32 products (launches back
+    to back behind a spin, one event pair): the BF16 tensor (512 MiB) rotates over 2 copies and the
+    FP8 codes (256 MiB) over 4, so every launch reads A from HBM like the 1 GiB FP32 tensor does;
+    the speedup divides the FP32 case's mean launch time by this one."""
     from oracle import oracle as O
-    stream = torch.cuda.current_stream()
     res = {"bf16": [], "fp8": []}
     sink = torch.zeros(1, dtype=torch.int32, device=Aflat.device)
-    flush = torch.empty(1 << 27, dtype=torch.float32, device=Aflat.device)   # 512 MiB > last-level cache
     Abf = ttv_amd.quantize_bf16(Aflat)            # same strides (flat)
+    Abfs = copies_for(Abf)
     abf_host = None
+    reps = max(12, args.steps)
     for c in cases:
         q, wa, pia = c["q"], c["wa"], c["pia"]
         b = c["b"]
@@ -550,19 +607,19 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
         err32 = float(np.linalg.norm(c32 - ref) / rn)
         for fmt in ("bf16", "fp8"):
             if fmt == "bf16":
-                Aq = Abf.as_strided(SHAPE, wa)
-                run = lambda: ttv_amd.ttv_bf16(q, Aq, b, out=Cq)  # noqa: E731
+                Cq = ttv_amd.ttv_bf16(q, Abf.as_strided(SHAPE, wa), b)
+                runs = [(lambda T=T.as_strided(SHAPE, wa): ttv_amd.ttv_bf16(q, T, b, out=Cq)) for T in Abfs]
                 extra = 0
-                Cq = ttv_amd.ttv_bf16(q, Aq, b)
                 elem = 2
             else:
                 codes, scales, fps = ttv_amd.quantize_fp8(q, Aflat.as_strided(SHAPE, wa))
                 Cq = ttv_amd.ttv_fp8(q, codes, scales, fps, b)
-                run = lambda: ttv_amd.ttv_fp8(q, codes, scales, fps, b, out=Cq)  # noqa: E731
+                flat = codes.as_strided([N], [1])
+                runs = [(lambda T=T.as_strided(SHAPE, wa): ttv_amd.ttv_fp8(q, T, scales, fps, b, out=Cq))
+                        for T in copies_for(flat)]
                 extra = scales.numel() * 4
                 elem = 1
-            ts = timed_launches(torch, ttv_amd, run, max(5, args.steps), sink, flush=flush)
-            ms = float(np.median(ts))          # speedups: medians of both (a rare stall skews a mean of ~50 us launches)
+            ms = timed_cycle(torch, ttv_amd, runs, reps, sink, us_per_launch=150.0)
             got = Cq.as_strided([Cq.numel()], [1]).cpu().numpy().astype(np.float64)
             err = float(np.linalg.norm(got - ref) / rn)
             # error of the accumulation alone: FP64 oracle on the dequantised values
@@ -573,23 +630,22 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
             else:
                 # scales are constant along each fiber: C = scale[j // fps] * sum_k decode(code) b_k
                 table = O.e4m3_decode(np.arange(256, dtype=np.uint8))
-                deq = table[codes.as_strided([N], [1]).cpu().numpy()]
+                deq = table[flat.cpu().numpy()]
                 sh = scales.cpu().numpy().astype(np.float64)
                 ref_q = O.naive(q, deq, SHAPE, wa, bh, pia=pia)
                 ref_q *= sh[np.arange(ref_q.size) // fps]
                 del deq
             err_acc = float(np.linalg.norm(got - ref_q) / np.linalg.norm(ref_q))
             nbytes = case_bytes(128, elem=elem, extra=extra)
-            res[fmt].append(dict(layout=c["layout"], q=q, ms=round(ms, 4), ms_mean=round(float(np.mean(ts)), 4),
-                                 speedup=round(c["ms_med"] / ms, 3),
-                                 gbs=round(nbytes / (ms * 1e-3) / 1e9, 1),
+            res[fmt].append(dict(layout=c["layout"], q=q, ms=round(ms, 4), speedup=round(c["ms"] / ms, 3),
+                                 gbs=round(nbytes / (ms * 1e-3) / 1e9, 1), frac=round(nbytes / (ms * 1e-3) / PEAK_BPS, 4),
                                  ideal_speedup=round(c["bytes"] / nbytes, 3),
                                  rel_err_vs_fp64=err, rel_err_vs_fp64_dequantised=err_acc, fp32_rel_err=err32))
             log("  %s %-5s q=%d %.4f ms  speedup %.2f (ideal %.2f)  rel err %.2e (accumulation %.2e)" % (
-                fmt, c["layout"], q, ms, c["ms_med"] / ms, c["bytes"] / nbytes, err, err_acc))
+                fmt, c["layout"], q, ms, c["ms"] / ms, c["bytes"] / nbytes, err, err_acc))
+            del runs, Cq
             if fmt == "fp8":
-                del codes, scales
-            del Cq
+                del codes, scales, flat
     summ = {}
     for fmt in ("bf16", "fp8"):
         sp = [r["speedup"] for r in res[fmt]]

@@ -253,6 +253,9 @@ int ttvx_batch_destroy(void* batch);
 /* Pure read stream over `bytes` (16-byte aligned) writing one float per block to `partial`
  * (partial_count >= number of blocks, at most 8 per CU): the practical HBM read ceiling. */
 int ttvx_bench_read(const void* src, size_t bytes, float* partial, size_t partial_count, void* stream);
+/* Reads src_bytes and writes dst_bytes (16-byte aligned) in one launch, each block its contiguous
+ * share of both: the practical ceiling of a product whose output is dst_bytes/src_bytes of its input. */
+int ttvx_bench_mix(const void* src, size_t src_bytes, void* dst, size_t dst_bytes, void* stream);
 /* Plain copy of `bytes` (16-byte aligned): a known-size kernel for counter calibration. */
 int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream);
 /* Bounded busy wait of one wave (iterations <= 100000 sleeps of ~8k cycles): keeps the queue

@@ -39,6 +39,38 @@ __global__ __launch_bounds__(kBlock) void k_read_stream(const u32x4* __restrict_
 	if(threadIdx.x == 0) out[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
+// Read stream plus a write stream of a fixed fraction, interleaved like a product's: block b streams
+// its contiguous chunk of src in rounds of U 16-byte loads per thread and, after each round, writes
+// the round's share of its contiguous chunk of dst with non-temporal 16-byte stores.
+__global__ __launch_bounds__(kBlock) void k_mix_stream(const u32x4* __restrict__ src, int64_t n16, int64_t chunk,
+                                                       u32x4* __restrict__ dst, int64_t m16, int64_t dchunk)
+{
+	constexpr int U = 8;
+	float s = 0.f;
+	int64_t const begin = static_cast<int64_t>(blockIdx.x) * chunk;
+	int64_t const end = begin + chunk < n16 ? begin + chunk : n16;
+	int64_t const dbegin = static_cast<int64_t>(blockIdx.x) * dchunk;
+	int64_t const dend = dbegin + dchunk < m16 ? dbegin + dchunk : m16;
+	int64_t const rounds = (end - begin + U * kBlock - 1) / (U * kBlock);
+	int64_t const dper = rounds > 0 ? (dend - dbegin + rounds - 1) / rounds : 0;    // stores per round
+	int64_t d = dbegin;
+	for(int64_t i = begin + threadIdx.x; i - threadIdx.x < end; i += U * kBlock) {
+		u32x4 v[U];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			int64_t const j = i + u * kBlock < end ? i + u * kBlock : end - 1;
+			v[u] = __builtin_nontemporal_load(src + j);
+		}
+#pragma unroll
+		for(int u = 0; u < U; ++u) s += __uint_as_float(v[u].x) + __uint_as_float(v[u].y) + __uint_as_float(v[u].z) + __uint_as_float(v[u].w);
+		int64_t const de = d + dper < dend ? d + dper : dend;
+		u32x4 w;
+		w.x = w.y = w.z = w.w = __float_as_uint(s);
+		for(int64_t j = d + threadIdx.x; j < de; j += kBlock) __builtin_nontemporal_store(w, dst + j);
+		d = de;
+	}
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_stream(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16)
 {
 	int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
@@ -68,6 +100,28 @@ int ttvx_bench_read(const void* src, size_t bytes, float* partial, size_t partia
 	                   static_cast<const u32x4*>(src), n16, chunk, partial);
 	hipError_t e = hipGetLastError();
 	return e == hipSuccess ? TTVX_OK : hip_fail(e, "read stream launch");
+}
+
+int ttvx_bench_mix(const void* src, size_t src_bytes, void* dst, size_t dst_bytes, void* stream)
+{
+	using namespace ttvx;
+	clear_error();
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	if(src_bytes % 16 != 0 || dst_bytes % 16 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0 ||
+	   reinterpret_cast<uintptr_t>(dst) % 16 != 0 || src_bytes == 0)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_mix: 16-byte aligned buffers and sizes required");
+	int rc = check_device_span(src, src_bytes, "src");
+	if(rc == TTVX_OK && dst_bytes > 0) rc = check_device_span(dst, dst_bytes, "dst");
+	if(rc != TTVX_OK) return rc;
+	// the read-stream grid (32 blocks per CU, 128 KiB chunks for 1 GiB)
+	int64_t const n16 = static_cast<int64_t>(src_bytes / 16), m16 = static_cast<int64_t>(dst_bytes / 16);
+	int64_t const grid = std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(di.cus) * 32, n16 / 256));
+	int64_t const chunk = (n16 + grid - 1) / grid, dchunk = (m16 + grid - 1) / grid;
+	hipLaunchKernelGGL(k_mix_stream, dim3(static_cast<unsigned>(grid)), dim3(kBlock), 0, static_cast<hipStream_t>(stream),
+	                   static_cast<const u32x4*>(src), n16, chunk, static_cast<u32x4*>(dst), m16, dchunk);
+	hipError_t e = hipGetLastError();
+	return e == hipSuccess ? TTVX_OK : hip_fail(e, "mix stream launch");
 }
 
 int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream)

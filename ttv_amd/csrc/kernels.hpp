@@ -206,6 +206,8 @@ struct Args {
 	// column outputs without reduction whose lane vectors exceed 16 bytes (BF16, FP8), a whole wave
 	// along i and 16-byte aligned C: stored wave-transposed (col_store_transposed)
 	int tstore;
+	// k_slab: log2(IT) and log2(IT*KS)
+	int itShift, spanShift;
 };
 
 // Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
@@ -280,6 +282,57 @@ __device__ __forceinline__ bool halving_sum(Acc (&v)[N], int lane, int stride, i
 	bool const store = d < 1 || ((lane / stride) & ((d << 1) - 1)) == 0;
 	for(; d >= 1; d >>= 1) v[0] += __shfl_xor(v[0], d * stride, 64);
 	return store;
+}
+
+// x from lane (lane ^ D): D < 32 through ds_swizzle in bit-mask mode (the lane index is an
+// immediate, no address arithmetic), D = 32 through __shfl_xor
+template<int D, typename Acc>
+__device__ __forceinline__ Acc xor_lane(Acc x)
+{
+	if constexpr(D < 32) {
+		constexpr int kPattern = 0x1f | (D << 10);     // and-mask 31, or-mask 0, xor-mask D
+		if constexpr(sizeof(Acc) == 4) {
+			return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), kPattern));
+		} else {
+			int const lo = __builtin_amdgcn_ds_swizzle(__double2loint(x), kPattern);
+			int const hi = __builtin_amdgcn_ds_swizzle(__double2hiint(x), kPattern);
+			return __hiloint2double(hi, lo);
+		}
+	} else {
+		return __shfl_xor(x, D, 64);
+	}
+}
+
+// halving_sum (stride 1) for a width W known at compile time: constant lane offsets
+template<typename Acc, int N, int W, int D = W / 2, int H = N / 2>
+__device__ __forceinline__ void halving_steps(Acc (&v)[N], int lane, int& base, int& n)
+{
+	if constexpr(H >= 1 && D >= 1) {
+		bool const up = (lane & D) != 0;
+#pragma unroll
+		for(int j = 0; j < H; ++j) {
+			Acc const send = up ? v[j] : v[j + H];
+			Acc const keep = up ? v[j + H] : v[j];
+			v[j] = keep + xor_lane<D>(send);
+		}
+		if(up) base += H;
+		n = H;
+		halving_steps<Acc, N, W, D / 2, H / 2>(v, lane, base, n);
+	} else if constexpr(D >= 1) {
+		v[0] += xor_lane<D>(v[0]);
+		halving_steps<Acc, N, W, D / 2, 0>(v, lane, base, n);
+	}
+}
+
+template<typename Acc, int N, int W>
+__device__ __forceinline__ bool halving_sum_ct(Acc (&v)[N], int lane, int& base, int& n)
+{
+	base = 0;
+	n = N;
+	halving_steps<Acc, N, W>(v, lane, base, n);
+	// the lanes that differ only in the bits below the last halving step hold the same sums
+	constexpr int kRest = W / N > 1 ? W / N : 1;       // lanes that end with identical sums
+	return (lane & (kRest - 1)) == 0;
 }
 
 // FP8 scales of the VEC consecutive outputs (o, i .. i+VEC-1), one group-index division per vector
@@ -908,6 +961,170 @@ __global__ __launch_bounds__(kBlock) void k_dot(const S* __restrict__ A, const t
 		}
 	} else {
 		for(int64_t g = g0; g < ge; g += gs) dot_group_long<S, VEC, U, KAHAN>(A, b, C, P, g, kb, ke);
+	}
+}
+
+// ---- one-wave-per-group kernels ----------------------------------------------------------------
+// Streaming A as one compact front: every wave (k_rows) or block (k_slab) reads one small work unit
+// (2-32 KiB), finishes it and exits, and the dispatcher hands out the next units in address order.
+// Per unit there is no loop and little set-up, so the kernels stay lean (the per-unit cost is
+// what limits this regime).  Measured on MI355X, FP32 128^4: rows of 128 elements 6.71 -> 7.02 TB/s
+// against row groups looped over a capped grid; 128 x 128 slabs in 256-byte column chunks 6.61 ->
+// 6.81 TB/s against whole-row items looped over a capped grid.
+
+// b[k0 .. k0+VEC) into registers, as 16-byte loads whatever b's alignment (global memory takes
+// element-aligned vector loads) when a lane's values fill whole 16-byte vectors
+typedef uint32_t u32x4_e __attribute__((ext_vector_type(4), aligned(4)));
+template<typename Acc, int VEC>
+__device__ __forceinline__ void load_b_vec(const Acc* __restrict__ b, int64_t k0, Acc (&bb)[VEC])
+{
+	constexpr int BYTES = VEC * static_cast<int>(sizeof(Acc));
+	if constexpr(BYTES % 16 == 0) {
+#pragma unroll
+		for(int c = 0; c < BYTES / 16; ++c) {
+			u32x4_e const q = *reinterpret_cast<const u32x4_e*>(reinterpret_cast<const unsigned char*>(b + k0) + 16 * c);
+			reinterpret_cast<uint32_t*>(bb)[4 * c]     = q.x;
+			reinterpret_cast<uint32_t*>(bb)[4 * c + 1] = q.y;
+			reinterpret_cast<uint32_t*>(bb)[4 * c + 2] = q.z;
+			reinterpret_cast<uint32_t*>(bb)[4 * c + 3] = q.w;
+		}
+	} else {
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) bb[e] = b[k0 + e];
+	}
+}
+
+// Rows of K <= 64*VEC contiguous elements (one vector per lane and row, P.kiter == 1): wave
+// blockIdx.x*4 + w sums the U*64/LPR rows of its group and stores their sums.  Every load is issued
+// before the first use and none sits behind a branch: lanes past the end of a row re-read the row's
+// first vector with weight zero.  LPRC > 0 fixes the lanes per row at compile time (constant
+// cross-lane offsets); LPRC == 0 reads it from P.LPR.
+template<typename S, int VEC, int U, int LPRC, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_rows(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                 void* __restrict__ C, Args P)
+{
+	using Acc = typename Traits<S>::acc;
+	int64_t const g = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+	if(g >= P.groups) return;
+	int const lpr   = LPRC > 0 ? LPRC : P.LPR;
+	int const shift = LPRC > 0 ? __builtin_ctz(LPRC) : P.lprShift;
+	int const rpw   = 64 >> shift;
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	int const r    = lane >> shift;
+	int64_t const k0 = static_cast<int64_t>(lane & (lpr - 1)) * VEC;
+	bool const live = k0 < P.K;          // VEC divides K: a lane's vector is wholly inside or outside
+	int64_t const kc = live ? k0 : 0;
+	Acc bb[VEC];
+	load_b_vec<Acc, VEC>(b, kc, bb);
+	// rows of the group: (g*U + u)*rpw + r; a group past the last row re-reads the last row
+	Vec<S, VEC> raw[U];
+	int64_t const row0 = g * U * rpw + r;
+#pragma unroll
+	for(int u = 0; u < U; ++u) {
+		int64_t const row = row0 + static_cast<int64_t>(u) * rpw;
+		raw[u].load(A + (row < P.O ? row : P.O - 1) * P.sAo + kc);
+	}
+	Acc v[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) {
+		Sum<Acc, KAHAN> acc;
+		Acc x[VEC];
+		decode(raw[u], x);
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
+		v[u] = live ? acc.value() : Acc(0);
+	}
+	int base, n;
+	bool store;
+	if constexpr(LPRC > 0) store = halving_sum_ct<Acc, U, LPRC>(v, lane, base, n);
+	else                   store = halving_sum<Acc, U>(v, lane, 1, lpr, base, n);
+	if(store) {
+#pragma unroll
+		for(int j = 0; j < U; ++j)
+			if(j < n) {
+				int64_t const row = row0 + static_cast<int64_t>(base + j) * rpw;
+				if(row < P.O) store_out<Acc, kScaled<S>, (sizeof(S) >= 4)>(C, P, row, 0, v[j]);
+			}
+	}
+}
+
+// Column items of one block each (col_narrow): item -> (OS slabs, chunk of IT*VEC columns); thread t
+// -> it = t % IT, ks = (t / IT) % KS (lane groups along k), os = t / (IT*KS), all of K; IT, KS, OS
+// powers of two with IT*KS*OS = 256.  Partial sums meet by shuffles inside a wave and, when one slab's
+// lanes span several waves, in LDS; the slab's first IT lanes store its IT*VEC outputs.
+template<typename S, int VEC, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_slab(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                 void* __restrict__ C, Args P)
+{
+	using Acc = typename Traits<S>::acc;
+	constexpr int KL = sizeof(S) >= 4 ? TTVX_KL_WIDE : TTVX_KL_NARROW;    // loads in flight per lane
+	__shared__ Acc red[kBlock / 64][64 * VEC];
+	int const t    = static_cast<int>(threadIdx.x);
+	int const span = P.IT * P.KS;                // lanes of one slab
+	int const it   = t & (P.IT - 1);
+	int const ks   = (t >> P.itShift) & (P.KS - 1);
+	int const os   = t >> P.spanShift;
+	// item -> (slab group, column chunk): blockIdx.x < 2^31, one 32-bit division at most
+	uint32_t const item = blockIdx.x;
+	uint32_t const q32 = P.iTiles == 1 ? item : item / static_cast<uint32_t>(P.iTiles);
+	int64_t const og = q32, itile = item - q32 * static_cast<uint32_t>(P.iTiles);
+	int64_t const o  = og * P.OS + os;
+	int64_t const oc = o < P.O ? o : P.O - 1;    // tail slabs and columns re-read live data, store nothing
+	int64_t const i  = (itile * P.IT + it) * VEC;
+	int64_t const ic = i < P.I ? i : P.I - VEC;
+	const S* pa = A + oc * P.sAo + static_cast<int64_t>(ks) * P.sAk + ic;
+	int64_t const kstep = static_cast<int64_t>(P.KS) * P.sAk;
+	Sum<Acc, KAHAN> acc[VEC];
+	// rounds of KL loads issued together; a lane's steps past its last row re-read the round's first
+	// row (and are not added), so short contractions keep all their loads in flight too
+	int64_t const nk = ks < P.K ? (P.K - 1 - ks) / P.KS + 1 : 0;
+	for(int64_t j0 = 0; j0 < nk; j0 += KL) {
+		Vec<S, VEC> raw[KL];
+		Acc bk[KL];
+#pragma unroll
+		for(int j = 0; j < KL; ++j) raw[j].load(pa + (j0 + j < nk ? j : 0) * kstep);
+#pragma unroll
+		for(int j = 0; j < KL; ++j) bk[j] = b[ks + (j0 + j < nk ? j0 + j : j0) * P.KS];
+		__builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+		for(int j = 0; j < KL; ++j) {
+			if(j0 + j < nk) {          // re-read rows are not added (a non-finite value must not leak)
+				Acc x[VEC];
+				decode(raw[j], x);
+#pragma unroll
+				for(int e = 0; e < VEC; ++e) acc[e].fma(x[e], bk[j]);
+			}
+		}
+		pa += KL * kstep;
+	}
+	Acc v[VEC];
+#pragma unroll
+	for(int e = 0; e < VEC; ++e) v[e] = acc[e].value();
+	// lanes it + IT*m of one wave hold partial sums of the same columns of the same slab
+	int const wspan = span < 64 ? span : 64;
+	for(int d = P.IT; d < wspan; d <<= 1)
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) v[e] += __shfl_xor(v[e], d, 64);
+	bool const live = o < P.O && i < P.I;
+	if(span <= 64) {
+		if(ks == 0 && live) store_vec<Acc, VEC, kScaled<S>>(C, P, o, i, v);
+		return;
+	}
+	// a slab's lanes span span/64 waves: their first IT lanes meet in LDS
+	int const lane = t & 63, w = t >> 6;
+	if(lane < P.IT)
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) red[w][lane * VEC + e] = v[e];
+	__syncthreads();
+	if((t & (span - 1)) < P.IT && live) {
+		int const w0 = w, gw = span / 64;
+#pragma unroll
+		for(int e = 0; e < VEC; ++e) {
+			Acc s = red[w0][it * VEC + e];
+			for(int ww = 1; ww < gw; ++ww) s += red[w0 + ww][it * VEC + e];
+			v[e] = s;
+		}
+		store_vec<Acc, VEC, kScaled<S>>(C, P, o, i, v);
 	}
 }
 

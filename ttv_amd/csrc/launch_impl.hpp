@@ -29,9 +29,43 @@ void launch_sched(Kern kernel, const Config& cfg, hipStream_t s, Xs... xs)
 	hipLaunchKernelGGL(kernel, dim3(gx, static_cast<unsigned>(cfg.ksplit)), dim3(kBlock), cfg.shmem, s, xs...);
 }
 
+// k_rows with the lanes per row fixed at compile time for full-width vectors (4..64 lanes), else read
+// at run time
+template<typename S, int VEC, int U, bool KH>
+int launch_rows(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
+{
+	dim3 const grid(static_cast<unsigned>(cfg.grid)), block(kBlock);
+	if constexpr(VEC == Traits<S>::vmax) {
+		switch(cfg.args.LPR) {
+			case 4:  hipLaunchKernelGGL((k_rows<S, VEC, U, 4, KH>), grid, block, 0, s, A, B, C, cfg.args); return TTVX_OK;
+			case 8:  hipLaunchKernelGGL((k_rows<S, VEC, U, 8, KH>), grid, block, 0, s, A, B, C, cfg.args); return TTVX_OK;
+			case 16: hipLaunchKernelGGL((k_rows<S, VEC, U, 16, KH>), grid, block, 0, s, A, B, C, cfg.args); return TTVX_OK;
+			case 32: hipLaunchKernelGGL((k_rows<S, VEC, U, 32, KH>), grid, block, 0, s, A, B, C, cfg.args); return TTVX_OK;
+			case 64: hipLaunchKernelGGL((k_rows<S, VEC, U, 64, KH>), grid, block, 0, s, A, B, C, cfg.args); return TTVX_OK;
+			default: break;
+		}
+	}
+	hipLaunchKernelGGL((k_rows<S, VEC, U, 0, KH>), grid, block, 0, s, A, B, C, cfg.args);
+	return TTVX_OK;
+}
+
 template<typename S, int VEC, bool KH>
 int launch_vec(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
 {
+	dim3 const grid(static_cast<unsigned>(cfg.grid)), block(kBlock);
+	if(cfg.lean == LEAN_SLAB) {
+		hipLaunchKernelGGL((k_slab<S, VEC, KH>), grid, block, 0, s, A, B, C, cfg.args);
+		return TTVX_OK;
+	}
+	if(cfg.lean == LEAN_ROWS) {
+		switch(cfg.U) {
+			case 1: return launch_rows<S, VEC, 1, KH>(cfg, A, B, C, s);
+			case 2: return launch_rows<S, VEC, 2, KH>(cfg, A, B, C, s);
+			case 4: return launch_rows<S, VEC, 4, KH>(cfg, A, B, C, s);
+			case 8: return launch_rows<S, VEC, 8, KH>(cfg, A, B, C, s);
+			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported row unroll");
+		}
+	}
 	if(cfg.kind == KIND_DOT) {
 		switch(cfg.U) {
 			case 1: launch_sched(k_dot<S, VEC, 1, KH>, cfg, s, A, B, C, cfg.args); break;

@@ -156,8 +156,9 @@ std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 //                             tiles), 1 = one contiguous range per block
 //   TTVX_GRID_PER_CU          block cap per CU of the column and strided launches
 //   TTVX_COL_WIDE_U           slab units per lane of the col_wide variant
-//   TTVX_DOT_GROUPS_PER_WAVE  row groups per wave the row launches aim at (twice that for 1- and
-//                             2-byte types, whose groups hold more rows)
+//   TTVX_DOT_GROUPS_PER_WAVE  row groups per wave the looping row launches aim at (twice that for
+//                             1- and 2-byte types, whose groups hold more rows)
+//   TTVX_NARROW_LANES         lanes per row chunk of col_narrow
 #ifndef TTVX_CONTIG_DEFAULT
 #define TTVX_CONTIG_DEFAULT 0
 #endif
@@ -178,6 +179,14 @@ constexpr int kGridPerCU = TTVX_GRID_PER_CU;
 #endif
 constexpr int kDotGroupsPerWave = TTVX_DOT_GROUPS_PER_WAVE;
 constexpr int kDotGridPerCU = 64;
+// largest grid of k_rows and k_slab (one work unit per wave or block)
+constexpr int64_t kMaxGrid = int64_t{1} << 30;
+// col_narrow's row chunk in lanes (16 lanes x 16-byte vectors = 256 bytes: 6.61 -> 6.81 TB/s on
+// FP32 slabs of 128 x 128 against whole 512-byte rows)
+#ifndef TTVX_NARROW_LANES
+#define TTVX_NARROW_LANES 16
+#endif
+constexpr int64_t kNarrowLanes = TTVX_NARROW_LANES;
 } // namespace
 
 bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus, unsigned flags, Config& cfg)
@@ -233,18 +242,25 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		P.kchunk = kchunk;
 		P.kiter  = static_cast<int>(std::min<int64_t>(1 << 30, ceil_div(std::min(kchunk, f.K), kstride)));
 		cfg.ksplit = static_cast<int>(ksplit);
-		// about kDotGroupsPerWave row groups per wave: enough waves to fill the GPU, and each wave
-		// still has a next group to load while it sums the current one
-		// (4 for the narrow types, whose groups hold more rows)
-		int64_t const dot_cap = static_cast<int64_t>(std::max(cus, 1)) * kDotGridPerCU;
-		int64_t const per_wave = dt.size >= 4 ? kDotGroupsPerWave : 2 * kDotGroupsPerWave;
-		cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, (kBlock / 64) * per_wave), dot_cap)));
+		if(P.kiter == 1 && ksplit == 1) {
+			// rows of one vector per lane: k_rows, one group per wave and no cap on the grid (short-lived
+			// blocks handed out in address order sweep A as one compact front)
+			cfg.lean = LEAN_ROWS;
+			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(P.groups, kBlock / 64), kMaxGrid)));
+		} else {
+			// about kDotGroupsPerWave row groups per wave (twice that for the narrow types, whose groups
+			// hold more rows): enough waves to fill the GPU, and each wave still has a next group to load
+			// while it sums the current one
+			int64_t const dot_cap = static_cast<int64_t>(std::max(cus, 1)) * kDotGridPerCU;
+			int64_t const per_wave = dt.size >= 4 ? kDotGroupsPerWave : 2 * kDotGroupsPerWave;
+			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, (kBlock / 64) * per_wave), dot_cap)));
+		}
 	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE) {
 		if(!col_shape) return false;
 		cfg.kind = KIND_COL;
 		cfg.vec  = pick_vec(dt, a_addr, f.I, {f.K > 1 ? f.sAk : 0, f.O > 1 ? f.sAo : 0});
 		int64_t const iv = f.I / cfg.vec;
-		int64_t const itmax = v == V_COL_NARROW ? 64 : kBlock;
+		int64_t const itmax = kBlock;
 		P.IT = static_cast<int>(std::min<int64_t>(iv, itmax));
 		// k groups per block: "wide" spreads the whole block over k (a flat walk through each
 		// slab); the other column variants keep >= 16 k steps per lane and fill the block with slabs
@@ -272,6 +288,26 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		} else {
 			P.iTiles = ceil_div(iv, P.IT);
 			P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS) * cfg.U) * P.iTiles;
+		}
+		if(v == V_COL_NARROW) {
+			// k_slab: rows cut into chunks of IT <= kNarrowLanes vectors (a power of two), the block's
+			// other lanes along k (as many as K uses), OS slabs per block, one item per block
+			P.IT = static_cast<int>(std::min<int64_t>(int64_t{1} << ilog2(iv), kNarrowLanes));
+			P.KS = static_cast<int>(std::min<int64_t>(kBlock / P.IT, pow2_ceil(f.K)));
+			P.OS = kBlock / (P.IT * P.KS);
+			P.itShift = ilog2(P.IT);
+			P.spanShift = ilog2(static_cast<int64_t>(P.IT) * P.KS);
+			cfg.U = 1;
+			P.along_i = 0;
+			P.iTiles = ceil_div(iv, P.IT);
+			P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS)) * P.iTiles;
+			P.red    = 2;
+			P.kchunk = f.K;
+			cfg.ksplit = 1;
+			cfg.lean = LEAN_SLAB;
+			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(P.items, kMaxGrid)));
+			if(fp64_final && dt.code != TTVX_F64) return false;      // no split partials in k_slab
+			return true;
 		}
 		int64_t ksplit = 1;
 		if(v == V_COL_SPLIT) {

@@ -54,8 +54,13 @@ struct DType {
 };
 DType dtype_of(int code);
 
+// Kernels of the one-wave-per-group regime (kernels.hpp): k_rows for rows of one vector per lane,
+// k_slab for col_narrow; LEAN_NONE for the looping k_dot / k_col / k_strided.
+enum Lean { LEAN_NONE = 0, LEAN_ROWS = 1, LEAN_SLAB = 2 };
+
 struct Config {
 	Kind kind = KIND_STRIDED;
+	Lean lean = LEAN_NONE;
 	int vec = 1;        // elements per load
 	int U = 1;
 	int variant = V_STRIDED;
