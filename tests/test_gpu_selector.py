@@ -1,6 +1,7 @@
 """Selector network kernels on the GPU against the FP64 oracle restatement (oracle.mlp_*):
 forward outputs and loss/gradient of the masked MSE; Adam decreases the loss; the GPU forward
-pass and the library's host forward pass pick the same variants."""
+pass and the library's host forward pass pick the same variants.  The shipped selection (tuning
+table, then network) is timed against every valid variant on 48 held-out shapes."""
 import numpy as np
 import pytest
 
@@ -72,3 +73,21 @@ def test_gpu_and_host_inference_agree(ttvx, device):
         ttvx.restore_default_selection()
     agree = np.mean(np.array(host) == choice)
     assert agree >= 0.99, agree
+
+
+def test_selection_regret_on_held_out_shapes(ttvx, device):
+    """48 shapes of the held-out set (seed 202; orders 2-10, 2^19..2^28 elements, all four storage
+    types), every valid variant timed HBM-cold on this GPU: the variants the default selection
+    picks are on average within 5% of the fastest, and no slower than the fixed rule's picks."""
+    from ttv_amd.selector import DT_CODE, collect_timings, regret
+    from ttv_amd.workloads import selector_shapes
+    shapes = selector_shapes(seed=202, count=48)
+    ttvx.restore_default_selection()
+    picks = np.array([ttvx.plan_info(s["q"], s["na"], s["wa"], s["pia"], DT_CODE[s["dtype"]])["variant"]
+                      for s in shapes])
+    _, times, heur = collect_timings(shapes, reps=5, log=None)
+    ttvx.restore_default_selection()
+    r_sel, r_fixed = regret(picks, times), regret(heur, times)
+    print("held-out regret: selection %.4f, fixed rule %.4f" % (r_sel, r_fixed), flush=True)
+    assert np.isfinite(r_sel) and r_sel <= 0.05, (r_sel, r_fixed)
+    assert r_sel <= r_fixed, (r_sel, r_fixed)

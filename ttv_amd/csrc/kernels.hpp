@@ -60,27 +60,34 @@ template<> struct Traits<uint8_t>  { using acc = float;  static constexpr int vm
 // against vmcnt.
 #define TTVX_GLOBAL(T, p) ((const __attribute__((address_space(1))) T*)(p))
 #define TTVX_GLOBAL_W(T, p) ((__attribute__((address_space(1))) T*)(p))
+// Vectors of A need only the element's alignment: global memory on gfx950 takes vector loads at
+// any byte alignment (rows whose length is not a multiple of the vector width keep 16-byte loads),
+// so the load types below promise no more than one byte of alignment.
+typedef uint32_t u32x4_b __attribute__((ext_vector_type(4), aligned(1)));
+typedef uint32_t u32x2_b __attribute__((ext_vector_type(2), aligned(1)));
+typedef uint32_t u32_b __attribute__((aligned(1)));
 template<int BYTES> struct RawLoad;
 template<> struct RawLoad<16> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[4]) {
-		u32x4 v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x4, p));
+		u32x4_b v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x4_b, p));
 		w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
 	}
 };
 template<> struct RawLoad<8> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[2]) {
-		u32x2 v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x2, p));
+		u32x2_b v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x2_b, p));
 		w[0] = v.x; w[1] = v.y;
 	}
 };
 template<> struct RawLoad<4> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(uint32_t, p));
+		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(u32_b, p));
 	}
 };
+typedef uint16_t u16_b __attribute__((aligned(1)));
 template<> struct RawLoad<2> {
 	static __device__ __forceinline__ void get(const void* p, uint32_t (&w)[1]) {
-		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(uint16_t, p));
+		w[0] = __builtin_nontemporal_load(TTVX_GLOBAL(u16_b, p));
 	}
 };
 template<> struct RawLoad<1> {
@@ -208,6 +215,8 @@ struct Args {
 	int tstore;
 	// k_slab: log2(IT) and log2(IT*KS)
 	int itShift, spanShift;
+	// rows (I == 1) whose length the vector width does not divide: their last vector is clamped
+	int kpart;
 };
 
 // Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
@@ -355,18 +364,23 @@ __device__ __forceinline__ void scale_vec(const Args& P, int64_t o, int64_t i, A
 	}
 }
 
-// VEC consecutive outputs (o, i .. i+VEC-1) of one lane: vector stores when C's alignment allows
-// (P.cvec) and no split partials are pending; FP8 scales with one group-index division per vector.
+// VEC consecutive outputs of one lane whose vector starts at column iu and was loaded from column
+// ic = col_clamp(iu) (ic < iu for the last, partial vector of a row whose length VEC does not
+// divide): vector stores when the vector is whole, C's alignment allows (P.cvec) and no split
+// partials are pending; FP8 scales with one group-index division per vector.
 // When a lane's vector is exactly one 16-byte store, consecutive lanes write consecutive 16-byte
 // pieces and the store is non-temporal; wider vectors (BF16, FP8) take several interleaved stores.
 template<typename Acc, int VEC, bool SC>
-__device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, int64_t o, int64_t i, const Acc (&v)[VEC])
+__device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, int64_t o, int64_t iu, int64_t ic,
+                                          const Acc (&v)[VEC])
 {
 	constexpr int BYTES = VEC * static_cast<int>(sizeof(Acc));
-	if(P.partial != nullptr || P.cvec == 0 || i + VEC > P.I || (BYTES >= 16 ? P.cvec < 16 : P.cvec < BYTES)) {
+	int64_t const i = iu;
+	if(ic != iu || P.partial != nullptr || P.cvec == 0 || i + VEC > P.I || (BYTES >= 16 ? P.cvec < 16 : P.cvec < BYTES)) {
+		// element stores; v[e] is column ic + e, and the columns before iu belong to the lane before
 #pragma unroll
 		for(int e = 0; e < VEC; ++e)
-			if(i + e < P.I) store_out<Acc, SC>(C, P, o, i + e, v[e]);
+			if(ic + e >= iu && ic + e < P.I) store_out<Acc, SC>(C, P, o, ic + e, v[e]);
 		return;
 	}
 	Acc w[VEC];
@@ -421,6 +435,13 @@ __device__ __forceinline__ void item_split(const Args& P, int64_t item, int64_t&
 // Thread t: it = t % IT, ks = (t / IT) % KS, os = t / (IT*KS).  Every thread of the block calls
 // col_item for the same item (the LDS reduction synchronises the block).
 // Unit u of a thread: slab o_u and first column i_u of its VEC-vector.
+// First column a lane's VEC-vector starting at column iu is loaded from: iu itself, or I - VEC for
+// the partial last vector of a row (and for lanes past the row, which store nothing); I >= VEC
+__device__ __forceinline__ int64_t col_clamp(const Args& P, int64_t iu, int VEC)
+{
+	return iu + VEC <= P.I ? iu : P.I - VEC;
+}
+
 __device__ __forceinline__ void col_unit(const Args& P, int64_t og, int64_t itile, int os, int it, int u, int U, int VEC,
                                          int64_t& o, int64_t& i)
 {
@@ -524,7 +545,7 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 		col_unit(P, og, itile, os, it, u, U, VEC, o[u], iu[u]);
 		// clamped: tail lanes re-read live data and store nothing
 		int64_t const oc = o[u] < P.O ? o[u] : P.O - 1;
-		int64_t const ic = iu[u] < P.I ? iu[u] : P.I - VEC;
+		int64_t const ic = col_clamp(P, iu[u], VEC);
 		pa[u] = A + oc * P.sAo + (kb + ks) * P.sAk + ic;
 	}
 	Sum<Acc, KAHAN> acc[U][VEC];
@@ -606,7 +627,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 							Acc x[VEC];
 #pragma unroll
 							for(int e = 0; e < VEC; ++e) x[e] = w[g * VEC + e];
-							if(oo < P.O && ii < P.I) store_vec<Acc, VEC, SC>(C, P, oo, ii, x);
+							if(oo < P.O && ii < P.I) store_vec<Acc, VEC, SC>(C, P, oo, ii, col_clamp(P, ii, VEC), x);
 						}
 					return;
 				}
@@ -615,10 +636,10 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 			for(int j = 0; j < U * VEC; ++j)
 				if(j < n) {
 					int const f = base + j;
-					int64_t oo, ii;
-					col_unit(P, og, itile, os, it, f / VEC, U, VEC, oo, ii);
-					ii += f % VEC;
-					if(oo < P.O && ii < P.I) store_out<Acc, SC>(C, P, oo, ii, w[j]);
+					int64_t oo, iu;
+					col_unit(P, og, itile, os, it, f / VEC, U, VEC, oo, iu);
+					int64_t const ii = col_clamp(P, iu, VEC) + f % VEC;
+					if(oo < P.O && ii >= iu && ii < P.I) store_out<Acc, SC>(C, P, oo, ii, w[j]);
 				}
 		}
 		return;
@@ -641,10 +662,10 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 			Acc s = Acc(0);
 			const Acc* src = red + (ur * VEC + e) * kBlock + orr * P.IT * P.KS + ir;
 			for(int q = 0; q < P.KS; ++q) s += src[q * P.IT];
-			int64_t oo, ii;
-			col_unit(P, og, itile, orr, ir, ur, U, VEC, oo, ii);
-			ii += e;
-			if(oo < P.O && ii < P.I) store_out<Acc, SC, true>(C, P, oo, ii, s);
+			int64_t oo, iu;
+			col_unit(P, og, itile, orr, ir, ur, U, VEC, oo, iu);
+			int64_t const ii = col_clamp(P, iu, VEC) + e;
+			if(oo < P.O && ii >= iu && ii < P.I) store_out<Acc, SC, true>(C, P, oo, ii, s);
 		}
 		__syncthreads();
 		return;
@@ -661,7 +682,7 @@ __device__ __forceinline__ void col_store(void* __restrict__ C, const Args& P, i
 		for(int u = 0; u < U; ++u) {
 			int64_t oo, ii;
 			col_unit(P, og, itile, os, it, u, U, VEC, oo, ii);
-			if(oo < P.O && ii < P.I) store_vec<Acc, VEC, SC>(C, P, oo, ii, v[u]);
+			if(oo < P.O && ii < P.I) store_vec<Acc, VEC, SC>(C, P, oo, ii, col_clamp(P, ii, VEC), v[u]);
 		}
 	}
 }
@@ -691,7 +712,7 @@ __device__ __forceinline__ void col_items_pipelined(const S* __restrict__ A, con
 			int64_t o, i;
 			col_unit(P, og, itile, os, it, u, U, VEC, o, i);
 			int64_t const oc = o < P.O ? o : P.O - 1;
-			int64_t const ic = i < P.I ? i : P.I - VEC;
+			int64_t const ic = col_clamp(P, i, VEC);
 			if(live) raw[u].load(A + oc * P.sAo + k * P.sAk + ic);
 			else
 #pragma unroll
@@ -748,6 +769,22 @@ __global__ __launch_bounds__(kBlock) void k_col(const S* __restrict__ A, const t
 // vector steps of b a lane keeps in registers for short rows: 16 values of b whatever the type
 __host__ __device__ constexpr int dot_breg(int vec) { return vec >= 16 ? 1 : (vec >= 8 ? 2 : 4); }
 
+// Start of the VEC elements loaded for a lane's vector at k (k < ke): k itself, or ke - VEC for the
+// partial last vector of a row whose length VEC does not divide (VEC <= the row length)
+__device__ __forceinline__ int64_t row_clamp(int64_t k, int64_t ke, int VEC)
+{
+	return k + VEC <= ke ? k : ke - VEC;
+}
+
+// Zeroes the decoded elements before position d: in a clamped vector they belong to the lane before
+// (a select, so that no value of another lane reaches the sum, finite or not)
+template<typename Acc, int VEC>
+__device__ __forceinline__ void mask_head(Acc (&x)[VEC], int64_t d)
+{
+#pragma unroll
+	for(int e = 0; e < VEC; ++e) x[e] = e < d ? Acc(0) : x[e];
+}
+
 template<typename S, int VEC>
 __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __restrict__ b, const Args& P, int64_t kb,
                                            int64_t ke, typename Traits<S>::acc (&bb)[dot_breg(VEC)][VEC])
@@ -757,12 +794,12 @@ __device__ __forceinline__ void dot_load_b(const typename Traits<S>::acc* __rest
 	int64_t const k0 = kb + static_cast<int64_t>(kl) * VEC;
 	int64_t const kstride = static_cast<int64_t>(P.LPR) * VEC;
 #pragma unroll
-	for(int s = 0; s < dot_breg(VEC); ++s)
+	for(int s = 0; s < dot_breg(VEC); ++s) {
+		int64_t const ks = k0 + s * kstride;
+		int64_t const kc = row_clamp(ks, ke, VEC);        // where the step's vector is loaded from
 #pragma unroll
-		for(int e = 0; e < VEC; ++e) {
-			int64_t const k = k0 + s * kstride + e;
-			bb[s][e] = (s < P.kiter && k < ke) ? b[k] : Acc(0);
-		}
+		for(int e = 0; e < VEC; ++e) bb[s][e] = (s < P.kiter && ks < ke) ? b[kc + e] : Acc(0);
+	}
 }
 
 // sums the lanes' partial row sums of group g and stores the U*RPW rows (non-temporal for 4- and
@@ -806,10 +843,11 @@ __device__ __forceinline__ void dot_rows_pipelined(const S* __restrict__ A, void
 	int const r    = lane >> P.lprShift;
 	int64_t const k0 = kb + static_cast<int64_t>(lane & (P.LPR - 1)) * VEC;
 	bool const live = k0 < ke;      // lanes past the end of the row contribute zero
+	int64_t const kc = live ? row_clamp(k0, ke, VEC) : k0;
 	auto load = [&](int64_t g, Vec<S, VEC> (&raw)[U]) {
 #pragma unroll
 		for(int u = 0; u < U; ++u) {
-			if(live) raw[u].load(dot_row_ptr<S, U>(A, P, g, u, r, k0));
+			if(live) raw[u].load(dot_row_ptr<S, U>(A, P, g, u, r, kc));
 			else
 #pragma unroll
 				for(int w = 0; w < Vec<S, VEC>::words; ++w) raw[u].w[w] = 0u;
@@ -822,6 +860,7 @@ __device__ __forceinline__ void dot_rows_pipelined(const S* __restrict__ A, void
 			Sum<Acc, KAHAN> acc;
 			Acc x[VEC];
 			decode(raw[u], x);
+			if(P.kpart) mask_head<Acc, VEC>(x, k0 - kc);
 #pragma unroll
 			for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
 			v[u] = acc.value();
@@ -857,14 +896,17 @@ __device__ __forceinline__ void dot_group_short(const S* __restrict__ A, void* _
 	Sum<Acc, KAHAN> acc[U];
 #pragma unroll
 	for(int s = 0; s < dot_breg(VEC); ++s) {
-		if(s < P.kiter && k0 + s * kstride < ke) {
+		int64_t const ks = k0 + s * kstride;
+		if(s < P.kiter && ks < ke) {
+			int64_t const kc = row_clamp(ks, ke, VEC);
 			Vec<S, VEC> raw[U];
 #pragma unroll
-			for(int u = 0; u < U; ++u) raw[u].load(pa[u] + s * kstride);
+			for(int u = 0; u < U; ++u) raw[u].load(pa[u] + (kc - k0));
 #pragma unroll
 			for(int u = 0; u < U; ++u) {
 				Acc x[VEC];
 				decode(raw[u], x);
+				if(P.kpart) mask_head<Acc, VEC>(x, ks - kc);
 #pragma unroll
 				for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bb[s][e]);
 			}
@@ -893,36 +935,41 @@ __device__ __forceinline__ void dot_group_long(const S* __restrict__ A, const ty
 	Sum<Acc, KAHAN> acc[U];
 	int64_t k = k0;
 	for(; k + kstride < ke; k += 2 * kstride) {
+		// the first step of the pair is whole; the second may be the row's partial last vector
+		int64_t const kk[2] = {k, row_clamp(k + kstride, ke, VEC)};
 		Vec<S, VEC> raw[2][U];
 		Acc bv[2][VEC];
 #pragma unroll
 		for(int h = 0; h < 2; ++h)
 #pragma unroll
-			for(int u = 0; u < U; ++u) raw[h][u].load(pa[u] + k + h * kstride);
+			for(int u = 0; u < U; ++u) raw[h][u].load(pa[u] + kk[h]);
 #pragma unroll
 		for(int h = 0; h < 2; ++h)
 #pragma unroll
-			for(int e = 0; e < VEC; ++e) bv[h][e] = b[k + h * kstride + e];
+			for(int e = 0; e < VEC; ++e) bv[h][e] = b[kk[h] + e];
 #pragma unroll
 		for(int h = 0; h < 2; ++h)
 #pragma unroll
 			for(int u = 0; u < U; ++u) {
 				Acc x[VEC];
 				decode(raw[h][u], x);
+				if(h == 1 && P.kpart) mask_head<Acc, VEC>(x, k + kstride - kk[1]);
 #pragma unroll
 				for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[h][e]);
 			}
 	}
 	if(k < ke) {
+		int64_t const kc = row_clamp(k, ke, VEC);
 		Acc bv[VEC];
 #pragma unroll
-		for(int e = 0; e < VEC; ++e) bv[e] = b[k + e];
+		for(int e = 0; e < VEC; ++e) bv[e] = b[kc + e];
 #pragma unroll
 		for(int u = 0; u < U; ++u) {
 			Vec<S, VEC> raw;
-			raw.load(pa[u] + k);
+			raw.load(pa[u] + kc);
 			Acc x[VEC];
 			decode(raw, x);
+			if(P.kpart) mask_head<Acc, VEC>(x, k - kc);
 #pragma unroll
 			for(int e = 0; e < VEC; ++e) acc[u].fma(x[e], bv[e]);
 		}
@@ -997,23 +1044,21 @@ __device__ __forceinline__ void load_b_vec(const Acc* __restrict__ b, int64_t k0
 // Rows of K <= 64*VEC contiguous elements (one vector per lane and row, P.kiter == 1): wave
 // blockIdx.x*4 + w sums the U*64/LPR rows of its group and stores their sums.  Every load is issued
 // before the first use and none sits behind a branch: lanes past the end of a row re-read the row's
-// first vector with weight zero.  LPRC > 0 fixes the lanes per row at compile time (constant
+// first vector and add nothing.  LPRC > 0 fixes the lanes per row at compile time (constant
 // cross-lane offsets); LPRC == 0 reads it from P.LPR.
 template<typename S, int VEC, int U, int LPRC, bool KAHAN>
-__global__ __launch_bounds__(kBlock) void k_rows(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
-                                                 void* __restrict__ C, Args P)
+__device__ __forceinline__ void rows_group(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                           void* __restrict__ C, const Args& P, int64_t g)
 {
 	using Acc = typename Traits<S>::acc;
-	int64_t const g = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
-	if(g >= P.groups) return;
 	int const lpr   = LPRC > 0 ? LPRC : P.LPR;
 	int const shift = LPRC > 0 ? __builtin_ctz(LPRC) : P.lprShift;
 	int const rpw   = 64 >> shift;
 	int const lane = static_cast<int>(threadIdx.x) & 63;
 	int const r    = lane >> shift;
 	int64_t const k0 = static_cast<int64_t>(lane & (lpr - 1)) * VEC;
-	bool const live = k0 < P.K;          // VEC divides K: a lane's vector is wholly inside or outside
-	int64_t const kc = live ? k0 : 0;
+	bool const live = k0 < P.K;
+	int64_t const kc = live ? row_clamp(k0, P.K, VEC) : 0;
 	Acc bb[VEC];
 	load_b_vec<Acc, VEC>(b, kc, bb);
 	// rows of the group: (g*U + u)*rpw + r; a group past the last row re-reads the last row
@@ -1030,6 +1075,7 @@ __global__ __launch_bounds__(kBlock) void k_rows(const S* __restrict__ A, const 
 		Sum<Acc, KAHAN> acc;
 		Acc x[VEC];
 		decode(raw[u], x);
+		if(P.kpart) mask_head<Acc, VEC>(x, k0 - kc);
 #pragma unroll
 		for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
 		v[u] = live ? acc.value() : Acc(0);
@@ -1048,30 +1094,37 @@ __global__ __launch_bounds__(kBlock) void k_rows(const S* __restrict__ A, const 
 	}
 }
 
+template<typename S, int VEC, int U, int LPRC, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_rows(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                 void* __restrict__ C, Args P)
+{
+	int64_t const g = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+	if(g < P.groups) rows_group<S, VEC, U, LPRC, KAHAN>(A, b, C, P, g);
+}
+
 // Column items of one block each (col_narrow): item -> (OS slabs, chunk of IT*VEC columns); thread t
 // -> it = t % IT, ks = (t / IT) % KS (lane groups along k), os = t / (IT*KS), all of K; IT, KS, OS
 // powers of two with IT*KS*OS = 256.  Partial sums meet by shuffles inside a wave and, when one slab's
 // lanes span several waves, in LDS; the slab's first IT lanes store its IT*VEC outputs.
 template<typename S, int VEC, bool KAHAN>
-__global__ __launch_bounds__(kBlock) void k_slab(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
-                                                 void* __restrict__ C, Args P)
+__device__ __forceinline__ void slab_item(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                          void* __restrict__ C, const Args& P, uint32_t item,
+                                          typename Traits<S>::acc (*red)[64 * VEC])
 {
 	using Acc = typename Traits<S>::acc;
 	constexpr int KL = sizeof(S) >= 4 ? TTVX_KL_WIDE : TTVX_KL_NARROW;    // loads in flight per lane
-	__shared__ Acc red[kBlock / 64][64 * VEC];
 	int const t    = static_cast<int>(threadIdx.x);
 	int const span = P.IT * P.KS;                // lanes of one slab
 	int const it   = t & (P.IT - 1);
 	int const ks   = (t >> P.itShift) & (P.KS - 1);
 	int const os   = t >> P.spanShift;
-	// item -> (slab group, column chunk): blockIdx.x < 2^31, one 32-bit division at most
-	uint32_t const item = blockIdx.x;
+	// item -> (slab group, column chunk): one 32-bit division at most
 	uint32_t const q32 = P.iTiles == 1 ? item : item / static_cast<uint32_t>(P.iTiles);
 	int64_t const og = q32, itile = item - q32 * static_cast<uint32_t>(P.iTiles);
 	int64_t const o  = og * P.OS + os;
 	int64_t const oc = o < P.O ? o : P.O - 1;    // tail slabs and columns re-read live data, store nothing
 	int64_t const i  = (itile * P.IT + it) * VEC;
-	int64_t const ic = i < P.I ? i : P.I - VEC;
+	int64_t const ic = col_clamp(P, i, VEC);
 	const S* pa = A + oc * P.sAo + static_cast<int64_t>(ks) * P.sAk + ic;
 	int64_t const kstep = static_cast<int64_t>(P.KS) * P.sAk;
 	Sum<Acc, KAHAN> acc[VEC];
@@ -1107,7 +1160,7 @@ __global__ __launch_bounds__(kBlock) void k_slab(const S* __restrict__ A, const 
 		for(int e = 0; e < VEC; ++e) v[e] += __shfl_xor(v[e], d, 64);
 	bool const live = o < P.O && i < P.I;
 	if(span <= 64) {
-		if(ks == 0 && live) store_vec<Acc, VEC, kScaled<S>>(C, P, o, i, v);
+		if(ks == 0 && live) store_vec<Acc, VEC, kScaled<S>>(C, P, o, i, ic, v);
 		return;
 	}
 	// a slab's lanes span span/64 waves: their first IT lanes meet in LDS
@@ -1124,8 +1177,17 @@ __global__ __launch_bounds__(kBlock) void k_slab(const S* __restrict__ A, const 
 			for(int ww = 1; ww < gw; ++ww) s += red[w0 + ww][it * VEC + e];
 			v[e] = s;
 		}
-		store_vec<Acc, VEC, kScaled<S>>(C, P, o, i, v);
+		store_vec<Acc, VEC, kScaled<S>>(C, P, o, i, ic, v);
 	}
+	__syncthreads();       // the next item of this block may write red
+}
+
+template<typename S, int VEC, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void k_slab(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
+                                                 void* __restrict__ C, Args P)
+{
+	__shared__ typename Traits<S>::acc red[kBlock / 64][64 * VEC];
+	slab_item<S, VEC, KAHAN>(A, b, C, P, blockIdx.x, red);
 }
 
 // ---- k_strided ------------------------------------------------------------------------------

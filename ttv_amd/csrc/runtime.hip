@@ -134,8 +134,18 @@ DType dtype_of(int code)
 
 namespace
 {
-// Widest power-of-two vector (elements) whose loads stay aligned for every index the kernel visits.
-int pick_vec(const DType& dt, uintptr_t a_addr, int64_t inner, std::initializer_list<int64_t> strides)
+// Vector width (elements, 16 bytes at most) of a row of `inner` elements: the widest power of two
+// that fits.  Vectors need only the element's alignment, and a row whose length the width does not
+// divide ends in a vector clamped to the row's end (kernels.hpp: col_clamp, row_clamp).
+int pick_vec(const DType& dt, int64_t inner)
+{
+	int v = dt.vmax;
+	while(v > 1 && v > inner) v >>= 1;
+	return v;
+}
+
+// Widest power-of-two vector whose loads are all naturally aligned (the selector's alignment feature).
+int aligned_vec(const DType& dt, uintptr_t a_addr, int64_t inner, std::initializer_list<int64_t> strides)
 {
 	int v = dt.vmax;
 	while(v > 1) {
@@ -222,7 +232,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 	if(v >= V_DOT_U1 && v <= V_DOT_SPLIT) {
 		if(!dot_shape) return false;
 		cfg.kind = KIND_DOT;
-		cfg.vec  = pick_vec(dt, a_addr, f.K, {f.O > 1 ? f.sAo : 0});
+		cfg.vec  = pick_vec(dt, f.K);
 		int64_t const vsteps = ceil_div(f.K, cfg.vec);
 		P.LPR = static_cast<int>(std::min<int64_t>(64, pow2_ceil(vsteps)));
 		P.lprShift = ilog2(P.LPR);
@@ -241,6 +251,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		ksplit = ceil_div(f.K, kchunk);
 		P.kchunk = kchunk;
 		P.kiter  = static_cast<int>(std::min<int64_t>(1 << 30, ceil_div(std::min(kchunk, f.K), kstride)));
+		P.kpart  = f.K % cfg.vec != 0 ? 1 : 0;
 		cfg.ksplit = static_cast<int>(ksplit);
 		if(P.kiter == 1 && ksplit == 1) {
 			// rows of one vector per lane: k_rows, one group per wave and no cap on the grid (short-lived
@@ -258,8 +269,8 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE) {
 		if(!col_shape) return false;
 		cfg.kind = KIND_COL;
-		cfg.vec  = pick_vec(dt, a_addr, f.I, {f.K > 1 ? f.sAk : 0, f.O > 1 ? f.sAo : 0});
-		int64_t const iv = f.I / cfg.vec;
+		cfg.vec  = pick_vec(dt, f.I);
+		int64_t const iv = ceil_div(f.I, cfg.vec);          // vectors per row (the last one clamped)
 		int64_t const itmax = kBlock;
 		P.IT = static_cast<int>(std::min<int64_t>(iv, itmax));
 		// k groups per block: "wide" spreads the whole block over k (a flat walk through each
@@ -378,7 +389,7 @@ int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned
 		if(configure(V_COL_SPLIT, f, dt, a_addr, cus, flags, cfg)) return V_COL_SPLIT;
 		configure(V_COL_U1, f, dt, a_addr, cus, flags, cfg);
 		if(small) return cfg.args.items >= cus ? V_COL_U1 : V_COL_NARROW;
-		int64_t const iv = f.I / cfg.vec;
+		int64_t const iv = ceil_div(f.I, cfg.vec);
 		// fp8: a 16-byte vector already holds 16 columns per lane
 		if(dt.size == 1) return iv >= 4 * kBlock ? V_COL_U2 : V_COL_NARROW;
 		if(iv >= 4 * kBlock) return V_COL_U4;            // wide rows: 4 vectors per lane along i
@@ -439,7 +450,7 @@ void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
 	out[8] = dt.code == TTVX_FP8E4M3 ? 1.f : 0.f;
 	out[9] = dt.code == TTVX_F64 ? 1.f : 0.f;
 	out[10] = f.I == 1 ? 1.f : 0.f;
-	out[11] = static_cast<float>(pick_vec(dt, a_addr, f.I > 1 ? f.I : f.K, {})) / static_cast<float>(dt.vmax);
+	out[11] = static_cast<float>(aligned_vec(dt, a_addr, f.I > 1 ? f.I : f.K, {})) / static_cast<float>(dt.vmax);
 	out[12] = f.three_d ? 1.f : 0.f;
 	out[13] = lg(static_cast<double>(N) * dt.size);
 	out[23] = static_cast<float>(f.qpos);
@@ -448,7 +459,7 @@ void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
 	Config c1, c2, c4;
 	int const cus = 256;
 	if(configure(V_COL_U1, f, dt, a_addr, cus, 0u, c1)) {
-		int64_t const iv = f.I / c1.vec;
+		int64_t const iv = ceil_div(f.I, c1.vec);
 		out[14] = lg(static_cast<double>(c1.args.items));
 		out[17] = static_cast<float>(static_cast<double>(iv) / static_cast<double>(c1.args.iTiles * c1.args.IT));
 		if(configure(V_COL_U2, f, dt, a_addr, cus, 0u, c2))
@@ -476,6 +487,29 @@ void features_of(const Fold& f, const DType& dt, uintptr_t a_addr, float* out)
 }
 
 // ---- launch ----------------------------------------------------------------------------------
+void set_output_stores(Config& cfg, const Fold& f, const DType& dt, uintptr_t c_addr)
+{
+	if(cfg.kind == KIND_COL) {
+		// largest power-of-two store (<= 16 bytes) that every lane's VEC outputs start aligned to
+		int64_t const ab = static_cast<int64_t>(acc_size(dt));
+		int chunk = 16;
+		while(chunk > static_cast<int>(ab) &&
+		      (c_addr % static_cast<uintptr_t>(chunk) != 0 || (f.O > 1 && (f.sCo * ab) % chunk != 0) ||
+		       (cfg.vec * ab) % chunk != 0))
+			chunk >>= 1;
+		cfg.args.cvec = chunk > static_cast<int>(ab) ? chunk : 0;
+		// wave-transposed stores: no reduction, a whole wave along i, lane vectors wider than 16
+		// bytes in an instantiation that carries them, 16-byte aligned C, no split partials
+		cfg.args.tstore = (cfg.args.red == 0 && cfg.args.IT % 64 == 0 && cfg.args.cvec >= 16 && f.I % cfg.vec == 0 &&
+		                   tstore_compiled(static_cast<int>(cfg.vec * ab), cfg.U) && cfg.ksplit == 1 &&
+		                   !cfg.args.part64) ? 1 : 0;
+		// their LDS staging: one run of 64*VEC outputs per wave
+		if(cfg.args.tstore)
+			cfg.shmem = std::max<std::size_t>(cfg.shmem, static_cast<std::size_t>(kBlock) * cfg.vec * static_cast<std::size_t>(ab));
+	}
+
+}
+
 namespace
 {
 // Split partial sums: one grow-only device buffer per (device, stream), reused in stream order by
@@ -527,25 +561,7 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 			return fail(TTVX_ERR_UNSUPPORTED, "ttvx: no kernel variant can run this problem");
 	}
 	bool const kahan = (flags & TTVX_FLAG_KAHAN) != 0;
-	if(cfg.kind == KIND_COL) {
-		// largest power-of-two store (<= 16 bytes) that every lane's VEC outputs start aligned to
-		int64_t const ab = static_cast<int64_t>(acc_size(dt));
-		uintptr_t const c_addr = reinterpret_cast<uintptr_t>(c);
-		int chunk = 16;
-		while(chunk > static_cast<int>(ab) &&
-		      (c_addr % static_cast<uintptr_t>(chunk) != 0 || (f.O > 1 && (f.sCo * ab) % chunk != 0) ||
-		       (cfg.vec * ab) % chunk != 0))
-			chunk >>= 1;
-		cfg.args.cvec = chunk > static_cast<int>(ab) ? chunk : 0;
-		// wave-transposed stores: no reduction, a whole wave along i, lane vectors wider than 16
-		// bytes in an instantiation that carries them, 16-byte aligned C, no split partials
-		cfg.args.tstore = (cfg.args.red == 0 && cfg.args.IT % 64 == 0 && cfg.args.cvec >= 16 &&
-		                   tstore_compiled(static_cast<int>(cfg.vec * ab), cfg.U) && cfg.ksplit == 1 &&
-		                   !cfg.args.part64) ? 1 : 0;
-		// their LDS staging: one run of 64*VEC outputs per wave
-		if(cfg.args.tstore)
-			cfg.shmem = std::max<std::size_t>(cfg.shmem, static_cast<std::size_t>(kBlock) * cfg.vec * static_cast<std::size_t>(ab));
-	}
+	set_output_stores(cfg, f, dt, reinterpret_cast<uintptr_t>(c));
 
 	if(cfg.kind == KIND_STRIDED) {
 		cfg.sargs.scales = scales;

@@ -75,6 +75,10 @@ struct Config {
 // Builds the launch configuration of variant v; false when v cannot run this problem.
 bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus, unsigned flags, Config& cfg);
 
+// Output-store form of a column configuration for C at c_addr: widest aligned store (args.cvec) and
+// wave-transposed stores (args.tstore, with their LDS staging in shmem).
+void set_output_stores(Config& cfg, const Fold& f, const DType& dt, uintptr_t c_addr);
+
 // Variant chosen by the fixed rule.
 int heuristic_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags);
 

@@ -133,13 +133,11 @@ def valid_variants(shape):
     return [v for v in vs if v != STRIDED] or vs
 
 
-def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0, repeated=False):
-    """Median time (ms) of every valid variant (NaN for invalid ones).
-
-    Default: every launch reads a region no previous launch touched (a workload streaming
-    through many tensors).  repeated=True: every launch reads the same tensor, with the
-    last-level cache evicted before each launch (the reference's benchmark protocol: repeated
-    products over one tensor, README.md:50)."""
+def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0):
+    """Median time (ms) of every valid variant (NaN for invalid ones).  Every launch reads a region
+    of the pool that no recent launch touched (at least 2 GiB rotate), so no launch finds its
+    operand in the 256 MB last-level cache: HBM-cold times of a workload streaming through many
+    tensors."""
     import torch
     import ttv_amd
     elem = ELEM[shape["dtype"]]
@@ -148,23 +146,18 @@ def time_shape(pool, shape, reps=3, spin_us_per_launch=120.0, repeated=False):
     vs = valid_variants(shape)
     out = np.full(NV, np.nan)
     stream = torch.cuda.current_stream()
-    fixed = pool.take(nbytes) if repeated else None
-    if repeated and getattr(pool, "flush", None) is None:
-        pool.flush = torch.empty(1 << 27, dtype=torch.float32, device=pool.A.device)   # 512 MiB
     # one untimed launch per variant (code objects resident, C allocated)
     for v in vs:
         ttv_amd.set_selection(ttv_amd.SELECT_FORCED, v)
-        run(fixed if repeated else pool.take(nbytes))
+        run(pool.take(nbytes))
     n_launch = len(vs) * reps
-    iters = int(min(100000, max(8, n_launch * (spin_us_per_launch + (100.0 if repeated else 0.0)) / 3.4)))
+    iters = int(min(100000, max(8, n_launch * spin_us_per_launch / 3.4)))
     check(lib().ttvx_bench_spin(iters, _ptr(pool.sink), _stream()))
     evs = []
     for v in vs:
         ttv_amd.set_selection(ttv_amd.SELECT_FORCED, v)
         for _ in range(reps):
-            if repeated:
-                pool.flush.zero_()
-            off = fixed if repeated else pool.take(nbytes)
+            off = pool.take(nbytes)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
             run(off)
@@ -190,7 +183,7 @@ def fixed_rule_variant(shape):
         ttv_amd.set_selection(ttv_amd.SELECT_AUTO)
 
 
-def collect_timings(shapes, reps=3, log=print, pool=None, repeated=False):
+def collect_timings(shapes, reps=3, log=print, pool=None):
     import ttv_amd
     pool = pool or TimingPool.for_shapes(shapes)
     feats = np.zeros((len(shapes), NF), np.float32)
@@ -201,7 +194,7 @@ def collect_timings(shapes, reps=3, log=print, pool=None, repeated=False):
         code = DT_CODE[s["dtype"]]
         feats[n] = ttv_amd.features(s["q"], s["na"], s["wa"], s["pia"], code)
         heur[n] = fixed_rule_variant(s)
-        times[n] = time_shape(pool, s, reps=reps, repeated=repeated)
+        times[n] = time_shape(pool, s, reps=reps)
         if log and (n + 1) % 500 == 0:
             log("timed %d/%d shapes in %.0f s" % (n + 1, len(shapes), time.time() - t0))
     return feats, times, heur
@@ -390,7 +383,7 @@ def main():
     ap.add_argument("--out", default="gpurun_out/selector")
     ap.add_argument("--tables", help="directory with timings_train.npz / timings_test.npz to train on")
     ap.add_argument("--retime-structured", action="store_true",
-                    help="with --tables: time the benchmark shapes again (repeated-tensor protocol) first")
+                    help="with --tables: time the benchmark shapes again first")
     a = ap.parse_args()
     import torch
     from .workloads import order8_shapes, selector_shapes, structured_shapes
@@ -413,7 +406,7 @@ def main():
         n_tr = len(f_tr) - n_struct
         if a.retime_structured:
             st_shapes = shapes_from_arrays(ztr)[n_tr:]
-            _, t_st, _ = collect_timings(st_shapes, reps=5, log=log, repeated=True)
+            _, t_st, _ = collect_timings(st_shapes, reps=5, log=log)
             t_tr[n_tr:] = t_st
             log("benchmark shapes timed again in %.0f s" % (time.time() - t0))
             np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
@@ -430,7 +423,7 @@ def main():
         np.savez_compressed(os.path.join(a.out, "timings_test.npz"), feats=f_te, times=t_te, heur=h_te,
                             **shape_arrays(shapes_te))
         f_r, t_r, h_r = collect_timings(shapes_tr[:n_tr], reps=a.reps, log=log, pool=pool)
-        f_s, t_s, h_s = collect_timings(shapes_tr[n_tr:], reps=5, log=log, pool=pool, repeated=True)
+        f_s, t_s, h_s = collect_timings(shapes_tr[n_tr:], reps=5, log=log, pool=pool)
         f_tr, t_tr, h_tr = np.concatenate([f_r, f_s]), np.concatenate([t_r, t_s]), np.concatenate([h_r, h_s])
         log("training timings done in %.0f s" % (time.time() - t0))
         np.savez_compressed(os.path.join(a.out, "timings_train.npz"), feats=f_tr, times=t_tr, heur=h_tr,
