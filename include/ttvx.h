@@ -239,7 +239,9 @@ typedef struct ttvx_batch_problem {
 } ttvx_batch_problem;
 
 /* Validates and plans `count` problems (device pointers, fixed for the batch's lifetime) and
- * returns an opaque handle. */
+ * returns an opaque handle.  Each problem keeps the tile configuration of its own selected
+ * variant; contractions longer than 2^20 and problems that fold to the strided kernel are
+ * rejected (TTVX_ERR_UNSUPPORTED). */
 int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** batch);
 /* Computes every problem of the batch in one persistent launch on `stream`; with use_graph != 0
  * the launch is captured into a hipGraph on first use and replayed afterwards.  A batch owns one
@@ -247,6 +249,11 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** b
 int ttvx_batch_launch(void* batch, int use_graph, void* stream);
 /* out4 = {tiles, algorithmic bytes, persistent grid size, problems}. */
 int ttvx_batch_info(void* batch, int64_t* out4);
+/* Kernel variant (0 .. ttvx_num_variants()-1) whose tiles run problem `index` of the batch: the
+ * variant a single call selects, with the split variants replaced by their unsplit forms
+ * (col_split -> col_u1, dot_split -> dot_u2).  A single call forced to it gives bit-identical
+ * outputs.  -1 on error. */
+int ttvx_batch_variant(void* batch, size_t index);
 int ttvx_batch_destroy(void* batch);
 
 /* ---- diagnostics (no counterpart in the reference; used by the benchmark) -------------- */

@@ -66,6 +66,7 @@ _SIGS = {
     "ttvx_batch_create": ([_sz, _vp, ctypes.POINTER(ctypes.c_void_p)], _int),
     "ttvx_batch_launch": ([_vp, _int, _vp], _int),
     "ttvx_batch_info": ([_vp, ctypes.POINTER(ctypes.c_int64)], _int),
+    "ttvx_batch_variant": ([_vp, _sz], _int),
     "ttvx_batch_destroy": ([_vp], _int),
 }
 

@@ -48,6 +48,13 @@ class Batch:
         check(lib().ttvx_batch_info(self._h, out))
         return dict(tiles=out[0], bytes=out[1], grid=out[2], problems=out[3])
 
+    def variant(self, index):
+        """Kernel variant running problem `index` (a single call forced to it is bit-identical)."""
+        v = lib().ttvx_batch_variant(self._h, index)
+        if v < 0:
+            check(_lib.ERR_ARGUMENT)
+        return v
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().ttvx_batch_destroy(self._h)
