@@ -21,7 +21,8 @@ Also reported on the same line:
   per_case      -- GB/s and fraction of peak for each (q, layout)
   read_ceiling  -- a pure 16-byte read stream over the same 1 GiB; mixed_ceilings -- the same read
                    stream plus a write stream of 1/128, 1/16 or 1/4 of it (the output fractions of
-                   128^4, 16^7 and the order-8 q = 4 product)
+                   128^4, 16^7 and the order-8 q = 4 product); each the faster of a persistent
+                   chunked stream and a compact sweep of short-lived blocks
   precision     -- BF16 and FP8-e4m3 storage of A: speedup over the FP32 kernel per case (both timed
                    with the protocol above) and the relative L2 error of C against the FP64 oracle on
                    the original data (and on the dequantised data)
@@ -338,22 +339,38 @@ def main():
                     algorithmic_bytes_per_launch=dom["bytes"], launch_ms=round(dom["ms"], 4))
 
     # practical ceilings: a pure read stream over the same buffer, and the same read stream with a
-    # write stream of 1/128, 1/16 and 1/4 of it (the output fractions of 128^4, 16^7 and order-8 q=4)
-    partial = torch.empty(256 * 32, dtype=torch.float32, device=dev)
+    # write stream of 1/128, 1/16 and 1/4 of it (the output fractions of 128^4, 16^7 and order-8 q=4);
+    # each the faster of two access patterns: one contiguous chunk per block (a persistent stream),
+    # and a compact sweep of short-lived blocks (2, 4 or 8 loads per thread, the best of the three)
+    partial = torch.empty(256 * 64, dtype=torch.float32, device=dev)
+    fsink = torch.zeros(1, dtype=torch.float32, device=dev)
     L = ttv_amd.lib()
-    read_ms = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.check(L.ttvx_bench_read(
-        Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(), stream.cuda_stream))], args.steps, sink)
-    read_gbs = N * 4 / (read_ms * 1e-3) / 1e9
-    log("read stream ceiling: %.1f GB/s (%.3f of peak)" % (read_gbs, read_gbs * 1e9 / PEAK_BPS))
     wbuf = torch.empty(N // 4, dtype=torch.float32, device=dev)
+
+    def ceiling(wbytes):
+        if wbytes == 0:
+            chunked = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.check(L.ttvx_bench_read(
+                Aflat.data_ptr(), N * 4, partial.data_ptr(), partial.numel(), stream.cuda_stream))], args.steps, sink)
+        else:
+            chunked = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.check(L.ttvx_bench_mix(
+                Aflat.data_ptr(), N * 4, wbuf.data_ptr(), wbytes, stream.cuda_stream))], args.steps, sink)
+        sweeps = [timed_cycle(torch, ttv_amd, [lambda u=u: ttv_amd.check(L.ttvx_bench_sweep(
+            Aflat.data_ptr(), N * 4, wbuf.data_ptr(), wbytes, u, fsink.data_ptr(), stream.cuda_stream))],
+            args.steps, sink) for u in (2, 4, 8)]
+        gbs = lambda ms: (N * 4 + wbytes) / (ms * 1e-3) / 1e9
+        best = min([chunked] + sweeps)
+        return {"gbs": round(gbs(best), 1), "frac": round(gbs(best) * 1e9 / PEAK_BPS, 4),
+                "chunked_gbs": round(gbs(chunked), 1), "sweep_gbs": round(gbs(min(sweeps)), 1)}
+
+    rc = ceiling(0)
+    read_gbs = rc["gbs"]
+    log("read ceiling: %.1f GB/s (%.3f of peak; chunked %.1f, sweep %.1f)" % (read_gbs, rc["frac"], rc["chunked_gbs"],
+                                                                            rc["sweep_gbs"]))
     mixed = {}
     for frac in (128, 16, 4):
-        wbytes = N * 4 // frac
-        ms = timed_cycle(torch, ttv_amd, [lambda: ttv_amd.check(L.ttvx_bench_mix(
-            Aflat.data_ptr(), N * 4, wbuf.data_ptr(), wbytes, stream.cuda_stream))], args.steps, sink)
-        gbs = (N * 4 + wbytes) / (ms * 1e-3) / 1e9
-        mixed["write_1_%d" % frac] = {"gbs": round(gbs, 1), "frac": round(gbs * 1e9 / PEAK_BPS, 4)}
-        log("read + 1/%d write ceiling: %.1f GB/s" % (frac, gbs))
+        mixed["write_1_%d" % frac] = m = ceiling(N * 4 // frac)
+        log("read + 1/%d write ceiling: %.1f GB/s (chunked %.1f, sweep %.1f)" % (frac, m["gbs"], m["chunked_gbs"],
+                                                                                m["sweep_gbs"]))
     del wbuf
 
     out = {
@@ -374,8 +391,7 @@ def main():
                    "shape": SHAPE, "elements": N, "bytes_per_product": case_bytes(128)},
         "frac_of_peak": round(value * 1e9 / PEAK_BPS, 4),
         "min_case_frac": round(min(pc["frac"] for pc in per_case), 4),
-        "read_ceiling": {"gbs": round(read_gbs, 1), "frac": round(read_gbs * 1e9 / PEAK_BPS, 4),
-                         "value_over_ceiling": round(value / read_gbs, 4)},
+        "read_ceiling": dict(rc, value_over_ceiling=round(value / read_gbs, 4)),
         "mixed_ceilings": mixed,
         "roofline": roofline,
         "per_case": per_case,
@@ -536,7 +552,8 @@ def other_configs_leg(torch, np, ttv_amd, Aflat, mixed, log, reps=10):
 
 def batching_leg(torch, np, ttv_amd, log, count=512, seed=4):
     """512 independent products (order 3/4, extents U-int[16,512], <= 2^26 elements each, random q
-    and layout) in one persistent hipGraph-replayed launch vs 512 sequential single launches."""
+    and layout) as one batch (one launch per tile body, replayed as a hipGraph) vs 512 sequential
+    single launches with the default selection."""
     from ttv_amd.batch import Batch
     from ttv_amd.workloads import batch_problems
     specs = batch_problems(seed=seed, count=count)
@@ -575,12 +592,13 @@ def batching_leg(torch, np, ttv_amd, log, count=512, seed=4):
         d = float(torch.linalg.vector_norm((Cb - Cs).double()) / torch.linalg.vector_norm(Cs.double()))
         worst = max(worst, d)
     gb = info["bytes"] / 1e9
-    res = dict(problems=count, tiles=info["tiles"], algorithmic_gb=round(gb, 2),
+    res = dict(problems=count, launches=info["launches"], blocks=info["blocks"], algorithmic_gb=round(gb, 2),
                batched_graph_ms=round(ms_graph, 3), batched_ms=round(ms_plain, 3), sequential_ms=round(ms_seq, 3),
                batched_gbs=round(gb / (ms_graph * 1e-3), 1), sequential_gbs=round(gb / (ms_seq * 1e-3), 1),
                speedup=round(ms_seq / ms_graph, 3), max_rel_diff_vs_sequential=worst)
-    log("batching: %d problems %.1f GB  graph %.3f ms (%.0f GB/s)  plain %.3f ms  sequential %.3f ms (%.0f GB/s)  "
-        "max rel diff %.1e" % (count, gb, ms_graph, gb / (ms_graph * 1e-3), ms_plain, ms_seq, gb / (ms_seq * 1e-3), worst))
+    log("batching: %d problems %.1f GB in %d launches  graph %.3f ms (%.0f GB/s)  plain %.3f ms  sequential %.3f ms "
+        "(%.0f GB/s)  speedup %.2f  max rel diff %.1e" % (count, gb, info["launches"], ms_graph, gb / (ms_graph * 1e-3),
+                                                         ms_plain, ms_seq, gb / (ms_seq * 1e-3), ms_seq / ms_graph, worst))
     bt.close()
     return res
 

@@ -243,11 +243,11 @@ typedef struct ttvx_batch_problem {
  * variant; contractions longer than 2^20 and problems that fold to the strided kernel are
  * rejected (TTVX_ERR_UNSUPPORTED). */
 int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** batch);
-/* Computes every problem of the batch in one persistent launch on `stream`; with use_graph != 0
- * the launch is captured into a hipGraph on first use and replayed afterwards.  A batch owns one
- * tile counter: launches of the same batch must not overlap (one stream, or ordered streams). */
+/* Computes every problem of the batch on `stream`: one launch per group of problems that share a
+ * tile body (a handful per batch); with use_graph != 0 the launches are captured into a hipGraph
+ * on first use and replayed afterwards.  Must be called on the device the batch was created on. */
 int ttvx_batch_launch(void* batch, int use_graph, void* stream);
-/* out4 = {tiles, algorithmic bytes, persistent grid size, problems}. */
+/* out4 = {blocks over all launches, algorithmic bytes, launches, problems}. */
 int ttvx_batch_info(void* batch, int64_t* out4);
 /* Kernel variant (0 .. ttvx_num_variants()-1) whose tiles run problem `index` of the batch: the
  * variant a single call selects, with the split variants replaced by their unsplit forms
@@ -257,12 +257,17 @@ int ttvx_batch_variant(void* batch, size_t index);
 int ttvx_batch_destroy(void* batch);
 
 /* ---- diagnostics (no counterpart in the reference; used by the benchmark) -------------- */
-/* Pure read stream over `bytes` (16-byte aligned) writing one float per block to `partial`
- * (partial_count >= number of blocks, at most 8 per CU): the practical HBM read ceiling. */
+/* Pure read stream over `bytes` (16-byte aligned), each block one contiguous chunk, writing one
+ * float per block to `partial` (blocks = min(partial_count, 64 per CU)). */
 int ttvx_bench_read(const void* src, size_t bytes, float* partial, size_t partial_count, void* stream);
 /* Reads src_bytes and writes dst_bytes (16-byte aligned) in one launch, each block its contiguous
- * share of both: the practical ceiling of a product whose output is dst_bytes/src_bytes of its input. */
+ * share of both (a product whose output is dst_bytes/src_bytes of its input). */
 int ttvx_bench_mix(const void* src, size_t src_bytes, void* dst, size_t dst_bytes, void* stream);
+/* The same traffic as a compact sweep: one short-lived block per loads_per_thread (2, 4 or 8) x 256
+ * 16-byte vectors of src, in address order, each writing its share of dst (dst_bytes may be 0;
+ * sink is one device float).  The benchmark's ceilings are the faster of the two patterns. */
+int ttvx_bench_sweep(const void* src, size_t src_bytes, void* dst, size_t dst_bytes, int loads_per_thread,
+                     float* sink, void* stream);
 /* Plain copy of `bytes` (16-byte aligned): a known-size kernel for counter calibration. */
 int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream);
 /* Bounded busy wait of one wave (iterations <= 100000 sleeps of ~8k cycles): keeps the queue

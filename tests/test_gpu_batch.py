@@ -78,7 +78,7 @@ def test_batch_matches_single_launches_and_oracle(ttvx, device):
     probs = _make(torch, device, specs, 6)
     bt = Batch(probs)
     info = bt.info()
-    assert info["problems"] == len(probs) and info["tiles"] > 0
+    assert info["problems"] == len(probs) and info["blocks"] > 0 and 1 <= info["launches"] <= len(probs)
     bt.launch(graph=False)
     torch.cuda.synchronize()
     batched = [_flat(C).cpu().numpy().copy() for C in bt.outputs]
@@ -104,7 +104,8 @@ def test_batch_full_workload(ttvx, device):
     probs = _make(torch, device, specs, 44)
     bt = Batch(probs)
     assert bt.info()["problems"] == len(probs)
-    print("batch: %d problems, %d tiles, grid %d" % (len(probs), bt.info()["tiles"], bt.info()["grid"]), flush=True)
+    print("batch: %d problems, %d blocks, %d launches" % (len(probs), bt.info()["blocks"], bt.info()["launches"]),
+          flush=True)
     bt.launch(graph=False)
     torch.cuda.synchronize()
     batched = [_flat(C).cpu().numpy().copy() for C in bt.outputs]

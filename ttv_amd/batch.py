@@ -1,7 +1,7 @@
 """Batched mode-q products in one persistent launch (ttvx_batch_* in ttvx.h).
 
     B = Batch([(q, A, b), ...])      # torch float32 GPU tensors; C tensors are allocated
-    B.launch(graph=True)             # one launch (captured into a hipGraph on first use)
+    B.launch(graph=True)             # one launch per tile body (captured into a hipGraph on first use)
     B.outputs                        # the C tensors, in problem order
 """
 import ctypes
@@ -46,7 +46,7 @@ class Batch:
     def info(self):
         out = (ctypes.c_int64 * 4)()
         check(lib().ttvx_batch_info(self._h, out))
-        return dict(tiles=out[0], bytes=out[1], grid=out[2], problems=out[3])
+        return dict(blocks=out[0], bytes=out[1], launches=out[2], problems=out[3])
 
     def variant(self, index):
         """Kernel variant running problem `index` (a single call forced to it is bit-identical)."""

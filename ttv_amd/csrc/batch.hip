@@ -1,24 +1,27 @@
-// Batched mode-q products: many independent TTVs of different order, shape, mode and layout in
-// one persistent launch (no counterpart in the reference, which runs one product per call).
+// Batched mode-q products: many independent TTVs of different order, shape, mode and layout in a
+// few launches (no counterpart in the reference, which runs one product per call).
 //
 // At creation every problem is validated and folded (plan.hpp), and the library's selection (the
 // tuning table, the learned selector, the fixed rule: the same choice a single call makes) picks its
-// kernel variant; the batch keeps that variant's tile configuration (column items of 1, 2 or 4
-// slab units, 256-byte column chunks, row groups of 1-8 row sets, all vector widths), with the
-// contraction never split.  A problem's work units (column items or row groups) are cut into batch
-// tiles of about kTileBytes of A each, so a tile costs one counter increment and one table search
-// however small its units are.  Problems are ordered largest first and their tiles numbered
-// consecutively.  The persistent kernel runs as many blocks as the GPU holds at once; each takes
-// tile numbers from a device counter (dynamic load balance), finds the owning problem by binary
-// search over the tile offsets and runs the tile body of the problem's variant, the same device
-// function the single-problem kernel runs, so a problem's result does not depend on whether it ran
-// batched or alone.  A fill kernel clears the counter before every launch; both kernels can be
-// captured once into a hipGraph and replayed.
+// kernel variant, with the contraction never split.  The problem keeps that variant's configuration
+// and its block decomposition: one block per four row groups (k_rows), per 256-byte column chunk
+// item (k_slab), per run of column items or row groups (k_col, k_dot) as the single launch's grid
+// divides them.  Problems whose blocks run the same tile body (body, vector width, slab units or row
+// sets, lanes per row) form a group, and a group is ONE launch whose grid is the concatenation of
+// its problems' grids: block b reads its problem from a per-block owner table, then runs the same
+// device function the single-problem kernel runs, so a problem's result does not depend on whether
+// it ran batched or alone.  Each group kernel has the register budget of its own body (a single
+// kernel holding every body would run at the occupancy of the largest), blocks stay short-lived
+// (the dispatcher balances the load), and a batch of hundreds of problems costs a handful of
+// launches instead of one launch, one ramp and one drain per problem.  The launches of a batch can
+// be captured once into a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
-#include <numeric>
+#include <map>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ttvx.h"
@@ -29,7 +32,7 @@
 namespace ttvx
 {
 
-// tile body of a problem
+// tile body of a group
 enum BatchBody { BB_COL = 0, BB_SLAB = 1, BB_ROWS = 2, BB_DOT = 3 };
 
 struct BatchDesc {
@@ -37,123 +40,156 @@ struct BatchDesc {
 	const float* a;
 	const float* b;
 	float* c;
-	int64_t units;     // column items, or row groups
-	int64_t per_tile;  // units per batch tile (a multiple of the block's waves for row groups)
-	int32_t body;      // BatchBody
-	int32_t U;         // slab units (BB_COL) or row sets (BB_ROWS, BB_DOT) per unit
-	int32_t vec;
-	int32_t variant;   // the selected variant (ttvx_batch_variant)
+	int64_t block0;    // first block of the problem in its group's grid
+	int64_t per;       // column items or row groups per block (BB_COL, BB_DOT)
 };
 
-constexpr int64_t kTileBytes = int64_t{1} << 20;
-constexpr int kBatchRed = kBlock * 4 * 4;      // LDS floats: column items of U*VEC <= 16 partials per lane
-
-__global__ __launch_bounds__(kBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
-
 template<int VEC, int U>
-__device__ __forceinline__ void batch_col(const BatchDesc& d, int64_t ub, int64_t ue, float* red)
+__device__ __forceinline__ void group_col(const BatchDesc& d, const Args& P, int64_t lb, float* red)
 {
-	if(d.args.one_step) {
-		col_items_pipelined<float, VEC, U>(d.a, d.b, d.c, d.args, ub, ue, 1, 0, d.args.K, red);
+	int64_t const ib = lb * d.per;
+	int64_t const ie = ib + d.per < P.items ? ib + d.per : P.items;
+	if(P.one_step) {
+		col_items_pipelined<float, VEC, U>(d.a, d.b, d.c, P, ib, ie, 1, 0, P.K, red);
 		return;
 	}
-	for(int64_t item = ub; item < ue; ++item)
-		col_item<float, VEC, U, false>(d.a, d.b, d.c, d.args, item, 0, d.args.K, red);
+	for(int64_t item = ib; item < ie; ++item) col_item<float, VEC, U, false>(d.a, d.b, d.c, P, item, 0, P.K, red);
 }
 
 template<int VEC, int U>
-__device__ __forceinline__ void batch_rows(const BatchDesc& d, int64_t gb, int64_t ge)
+__device__ __forceinline__ void group_dot(const BatchDesc& d, const Args& P, int64_t lb)
 {
-	for(int64_t g = gb + (threadIdx.x >> 6); g < ge; g += kBlock / 64)
-		rows_group<float, VEC, U, 0, false>(d.a, d.b, d.c, d.args, g);
-}
-
-template<int VEC, int U>
-__device__ __forceinline__ void batch_dot(const BatchDesc& d, int64_t gb, int64_t ge)
-{
-	if(d.args.kiter <= dot_breg(VEC)) {
+	int64_t const gb = lb * d.per;
+	int64_t const ge = gb + d.per < P.groups ? gb + d.per : P.groups;
+	int64_t const g0 = gb + (threadIdx.x >> 6);
+	if(P.kiter <= dot_breg(VEC)) {
 		float bb[dot_breg(VEC)][VEC];
-		dot_load_b<float, VEC>(d.b, d.args, 0, d.args.K, bb);
-		for(int64_t g = gb + (threadIdx.x >> 6); g < ge; g += kBlock / 64)
-			dot_group_short<float, VEC, U, false>(d.a, d.c, d.args, g, 0, d.args.K, bb);
+		dot_load_b<float, VEC>(d.b, P, 0, P.K, bb);
+		for(int64_t g = g0; g < ge; g += kBlock / 64) dot_group_short<float, VEC, U, false>(d.a, d.c, P, g, 0, P.K, bb);
 	} else {
-		for(int64_t g = gb + (threadIdx.x >> 6); g < ge; g += kBlock / 64)
-			dot_group_long<float, VEC, U, false>(d.a, d.b, d.c, d.args, g, 0, d.args.K);
+		for(int64_t g = g0; g < ge; g += kBlock / 64) dot_group_long<float, VEC, U, false>(d.a, d.b, d.c, P, g, 0, P.K);
 	}
 }
 
-template<int VEC>
-__device__ __forceinline__ void batch_tile(const BatchDesc& d, int64_t ub, int64_t ue, float* red)
+// One launch of a group: block b runs local block b - block0 of problem owner[b].
+template<int BODY, int VEC, int U, int LPRC>
+__global__ __launch_bounds__(kBlock) void k_batch_group(const BatchDesc* __restrict__ descs,
+                                                        const uint32_t* __restrict__ owner)
 {
-	switch(d.body) {
-		case BB_COL:
-			if(d.U == 1) batch_col<VEC, 1>(d, ub, ue, red);
-			else if(d.U == 2) batch_col<VEC, 2>(d, ub, ue, red);
-			else batch_col<VEC, 4>(d, ub, ue, red);
-			break;
-		case BB_SLAB:
-			for(int64_t item = ub; item < ue; ++item)
-				slab_item<float, VEC, false>(d.a, d.b, d.c, d.args, static_cast<uint32_t>(item),
-				                             reinterpret_cast<float (*)[64 * VEC]>(red));
-			break;
-		case BB_ROWS:
-			if(d.U == 1) batch_rows<VEC, 1>(d, ub, ue);
-			else if(d.U == 2) batch_rows<VEC, 2>(d, ub, ue);
-			else if(d.U == 4) batch_rows<VEC, 4>(d, ub, ue);
-			else batch_rows<VEC, 8>(d, ub, ue);
-			break;
-		default:
-			if(d.U == 1) batch_dot<VEC, 1>(d, ub, ue);
-			else if(d.U == 2) batch_dot<VEC, 2>(d, ub, ue);
-			else if(d.U == 4) batch_dot<VEC, 4>(d, ub, ue);
-			else batch_dot<VEC, 8>(d, ub, ue);
-			break;
+	BatchDesc const& d = descs[owner[blockIdx.x]];
+	Args const P = d.args;
+	int64_t const lb = static_cast<int64_t>(blockIdx.x) - d.block0;
+	if constexpr(BODY == BB_ROWS) {
+		int64_t const g = lb * (kBlock / 64) + (threadIdx.x >> 6);
+		if(g < P.groups) rows_group<float, VEC, U, LPRC, false>(d.a, d.b, d.c, P, g);
+	} else if constexpr(BODY == BB_SLAB) {
+		__shared__ float red[kBlock / 64][64 * VEC];
+		slab_item<float, VEC, false>(d.a, d.b, d.c, P, static_cast<uint32_t>(lb), red);
+	} else if constexpr(BODY == BB_COL) {
+		extern __shared__ unsigned char smem_raw[];
+		group_col<VEC, U>(d, P, lb, reinterpret_cast<float*>(smem_raw));
+	} else {
+		group_dot<VEC, U>(d, P, lb);
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchDesc* __restrict__ descs, const int64_t* __restrict__ tile0,
-                                                  int nprob, int64_t total_tiles, unsigned int* __restrict__ counter)
+namespace
 {
-	__shared__ float red[kBatchRed];
-	__shared__ int64_t s_tile;
-	for(;;) {
-		if(threadIdx.x == 0) s_tile = static_cast<int64_t>(atomicAdd(counter, 1u));
-		__syncthreads();
-		int64_t const t = s_tile;
-		__syncthreads();
-		if(t >= total_tiles) break;
-		// owning problem: last table entry with tile0 <= t
-		int lo = 0, hi = nprob - 1;
-		while(lo < hi) {
-			int const mid = (lo + hi + 1) >> 1;
-			if(tile0[mid] <= t) lo = mid;
-			else hi = mid - 1;
+using GroupKernel = void (*)(const BatchDesc*, const uint32_t*);
+
+// group key: body, vector width, slab units / row sets, lanes per row fixed at compile time
+using Key = std::tuple<int, int, int, int>;
+
+template<int VEC, int U>
+GroupKernel rows_kernel(int lprc)
+{
+	// compile-time lanes per row for full-width vectors, as the single-problem k_rows
+	if constexpr(VEC == 4) {
+		switch(lprc) {
+			case 4:  return k_batch_group<BB_ROWS, VEC, U, 4>;
+			case 8:  return k_batch_group<BB_ROWS, VEC, U, 8>;
+			case 16: return k_batch_group<BB_ROWS, VEC, U, 16>;
+			case 32: return k_batch_group<BB_ROWS, VEC, U, 32>;
+			case 64: return k_batch_group<BB_ROWS, VEC, U, 64>;
+			default: return k_batch_group<BB_ROWS, VEC, U, 0>;
 		}
-		BatchDesc const& d = descs[lo];
-		int64_t const ub = (t - tile0[lo]) * d.per_tile;
-		int64_t const ue = ub + d.per_tile < d.units ? ub + d.per_tile : d.units;
-		switch(d.vec) {
-			case 4:  batch_tile<4>(d, ub, ue, red); break;
-			case 2:  batch_tile<2>(d, ub, ue, red); break;
-			default: batch_tile<1>(d, ub, ue, red); break;
+	} else {
+		return k_batch_group<BB_ROWS, VEC, U, 0>;
+	}
+}
+
+template<int BODY, int VEC>
+GroupKernel kernel_u(int U, int lprc)
+{
+	if constexpr(BODY == BB_SLAB) {
+		return U == 1 ? k_batch_group<BB_SLAB, VEC, 1, 0> : nullptr;
+	} else if constexpr(BODY == BB_COL) {
+		switch(U) {
+			case 1: return k_batch_group<BB_COL, VEC, 1, 0>;
+			case 2: return k_batch_group<BB_COL, VEC, 2, 0>;
+			case 4: return k_batch_group<BB_COL, VEC, 4, 0>;
+			default: return nullptr;
+		}
+	} else if constexpr(BODY == BB_DOT) {
+		switch(U) {
+			case 1: return k_batch_group<BB_DOT, VEC, 1, 0>;
+			case 2: return k_batch_group<BB_DOT, VEC, 2, 0>;
+			case 4: return k_batch_group<BB_DOT, VEC, 4, 0>;
+			case 8: return k_batch_group<BB_DOT, VEC, 8, 0>;
+			default: return nullptr;
+		}
+	} else {
+		switch(U) {
+			case 1: return rows_kernel<VEC, 1>(lprc);
+			case 2: return rows_kernel<VEC, 2>(lprc);
+			case 4: return rows_kernel<VEC, 4>(lprc);
+			case 8: return rows_kernel<VEC, 8>(lprc);
+			default: return nullptr;
 		}
 	}
 }
+
+template<int BODY>
+GroupKernel kernel_vec(int vec, int U, int lprc)
+{
+	switch(vec) {
+		case 1: return kernel_u<BODY, 1>(U, lprc);
+		case 2: return kernel_u<BODY, 2>(U, lprc);
+		case 4: return kernel_u<BODY, 4>(U, lprc);
+		default: return nullptr;
+	}
+}
+
+GroupKernel group_kernel(Key const& k)
+{
+	int const body = std::get<0>(k), vec = std::get<1>(k), U = std::get<2>(k), lprc = std::get<3>(k);
+	switch(body) {
+		case BB_COL:  return kernel_vec<BB_COL>(vec, U, lprc);
+		case BB_SLAB: return kernel_vec<BB_SLAB>(vec, U, lprc);
+		case BB_ROWS: return kernel_vec<BB_ROWS>(vec, U, lprc);
+		default:      return kernel_vec<BB_DOT>(vec, U, lprc);
+	}
+}
+} // namespace
+
+struct BatchLaunch {
+	GroupKernel kernel = nullptr;
+	int64_t blocks = 0;
+	std::size_t shmem = 0;
+	std::size_t desc_off = 0;    // byte offsets into the batch's device memory
+	std::size_t owner_off = 0;
+};
 
 struct Batch {
 	int device = -1;
 	int nprob = 0;
-	int64_t total_tiles = 0;
+	int64_t total_blocks = 0;
 	int64_t bytes = 0;
-	int grid = 0;
-	void* dmem = nullptr;              // descriptors followed by the tile counter
-	BatchDesc* descs = nullptr;
-	int64_t* tile0 = nullptr;
-	unsigned int* counter = nullptr;
+	void* dmem = nullptr;              // descriptors and owner tables of every launch
+	std::vector<BatchLaunch> launches;
 	hipStream_t capture_stream = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
 	hipGraph_t graph = nullptr;
-	std::vector<int> order;            // descriptor slot -> caller's problem index
 	std::vector<int> variant;          // caller's problem index -> variant of its tiles
 };
 
@@ -161,11 +197,15 @@ namespace
 {
 int enqueue(Batch& bt, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, bt.counter);
-	hipLaunchKernelGGL(k_batch, dim3(static_cast<unsigned>(bt.grid)), dim3(kBlock), 0, s, bt.descs, bt.tile0,
-	                   bt.nprob, bt.total_tiles, bt.counter);
-	hipError_t e = hipGetLastError();
-	return e == hipSuccess ? TTVX_OK : hip_fail(e, "batch launch");
+	auto* base = static_cast<char*>(bt.dmem);
+	for(BatchLaunch const& L : bt.launches) {
+		hipLaunchKernelGGL(L.kernel, dim3(static_cast<unsigned>(L.blocks)), dim3(kBlock), L.shmem, s,
+		                   reinterpret_cast<const BatchDesc*>(base + L.desc_off),
+		                   reinterpret_cast<const uint32_t*>(base + L.owner_off));
+		hipError_t e = hipGetLastError();
+		if(e != hipSuccess) return hip_fail(e, "batch launch");
+	}
+	return TTVX_OK;
 }
 } // namespace
 
@@ -181,18 +221,21 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 	if(out == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx_batch_create: null output handle");
 	*out = nullptr;
 	if(count == 0 || problems == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx_batch_create: empty batch");
+	if(count > (size_t{1} << 31)) return fail(TTVX_ERR_ARGUMENT, "ttvx_batch_create: too many problems");
 	DeviceInfo di;
 	if(!current_device(di)) return TTVX_ERR_DEVICE;
 	DType const dt = dtype_of(TTVX_F32);
 
 	struct Item {
 		BatchDesc d;
-		int64_t tiles;
+		Key key;
+		int64_t blocks;
+		std::size_t shmem;
 		int64_t cost;
-		int index;
 	};
 	std::vector<Item> items;
 	items.reserve(count);
+	std::vector<int> variant(count, -1);
 	int64_t bytes = 0;
 	for(size_t n = 0; n < count; ++n) {
 		ttvx_batch_problem const& pr = problems[n];
@@ -219,27 +262,29 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 			return fail(TTVX_ERR_UNSUPPORTED, where + "the batched kernel needs A and C to fold to three dense-inner dimensions");
 		Item it;
 		std::memset(&it, 0, sizeof(it));
-		int64_t unit_bytes;
+		int body, lprc = 0;
 		if(cfg.lean == LEAN_SLAB) {
-			it.d.body  = BB_SLAB;
-			it.d.units = cfg.args.items;
-			unit_bytes = static_cast<int64_t>(cfg.args.OS) * cfg.args.IT * cfg.vec * f.K * 4;
+			body = BB_SLAB;
+			it.blocks = cfg.args.items;
 		} else if(cfg.kind == KIND_COL) {
-			it.d.body  = BB_COL;
-			it.d.units = cfg.args.items;
-			unit_bytes = static_cast<int64_t>(cfg.args.OS) * cfg.U * cfg.args.IT * cfg.vec * f.K * 4;
+			body = BB_COL;
+			it.d.per = std::max<int64_t>(1, (cfg.args.items + cfg.grid - 1) / cfg.grid);
+			it.blocks = (cfg.args.items + it.d.per - 1) / it.d.per;
+		} else if(cfg.lean == LEAN_ROWS) {
+			body = BB_ROWS;
+			it.blocks = (cfg.args.groups + kBlock / 64 - 1) / (kBlock / 64);
+			if(cfg.vec == 4 && cfg.args.LPR >= 4) lprc = cfg.args.LPR;
 		} else {
-			it.d.body  = cfg.lean == LEAN_ROWS ? BB_ROWS : BB_DOT;
-			it.d.units = cfg.args.groups;
-			unit_bytes = static_cast<int64_t>(64 / cfg.args.LPR) * cfg.U * f.K * 4;
+			body = BB_DOT;
+			int64_t const waves = kBlock / 64;
+			int64_t const per = std::max<int64_t>(1, (cfg.args.groups + cfg.grid - 1) / cfg.grid);
+			it.d.per = (per + waves - 1) / waves * waves;
+			it.blocks = (cfg.args.groups + it.d.per - 1) / it.d.per;
 		}
-		if(cfg.vec > 4 || cfg.shmem > sizeof(float) * kBatchRed || (cfg.lean == LEAN_SLAB && 4 * 64 * cfg.vec > kBatchRed))
-			return fail(TTVX_ERR_UNSUPPORTED, where + "tile does not fit the batched kernel");
-		it.d.per_tile = std::max<int64_t>(1, kTileBytes / std::max<int64_t>(unit_bytes, 1));
-		if(it.d.body == BB_ROWS || it.d.body == BB_DOT)
-			it.d.per_tile = (it.d.per_tile + kBlock / 64 - 1) / (kBlock / 64) * (kBlock / 64);
-		it.tiles = (it.d.units + it.d.per_tile - 1) / it.d.per_tile;
-		it.cost  = f.outputs * f.K;
+		it.key = Key(body, cfg.vec, cfg.U, lprc);
+		if(group_kernel(it.key) == nullptr || cfg.shmem > 64 * 1024 || it.blocks > kMaxLeanBlocks)
+			return fail(TTVX_ERR_UNSUPPORTED, where + "tile does not fit the batched kernels");
+		it.shmem = body == BB_COL ? cfg.shmem : 0;
 		set_output_stores(cfg, f, dt, reinterpret_cast<uintptr_t>(pr.c));
 		it.d.args = cfg.args;
 		it.d.args.kchunk  = f.K;
@@ -248,56 +293,85 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 		it.d.args.scales  = nullptr;
 		it.d.args.fps     = 1;
 		it.d.args.tstore  = 0;
-		it.d.U = cfg.U;
-		it.d.variant = v;
-		it.d.vec = cfg.vec;
 		it.d.a = pr.a;
 		it.d.b = pr.b;
 		it.d.c = pr.c;
-		it.index = static_cast<int>(n);
+		it.cost = f.outputs * f.K;
+		variant[n] = v;
 		items.push_back(it);
 		bytes += f.outputs * f.K * 4 + f.outputs * 4 + f.K * 4;
 	}
-	std::stable_sort(items.begin(), items.end(), [](Item const& x, Item const& y) { return x.cost > y.cost; });
 
+	// groups: problems of one tile body, largest first inside a group (the small ones fill the
+	// drain), groups launched largest first; a group whose grid would pass the launch limit
+	// continues in another launch
+	std::map<Key, std::vector<int>> groups;
+	for(size_t n = 0; n < items.size(); ++n) groups[items[n].key].push_back(static_cast<int>(n));
+	struct Plan {
+		Key key;
+		std::vector<int> members;
+		int64_t blocks = 0;
+		int64_t cost = 0;
+		std::size_t shmem = 0;
+	};
+	std::vector<Plan> plans;
+	for(auto& g : groups) {
+		std::stable_sort(g.second.begin(), g.second.end(),
+		                 [&](int x, int y) { return items[x].cost > items[y].cost; });
+		Plan cur;
+		cur.key = g.first;
+		for(int m : g.second) {
+			if(cur.blocks + items[m].blocks > kMaxLeanBlocks) {
+				plans.push_back(cur);
+				cur = Plan{};
+				cur.key = g.first;
+			}
+			items[m].d.block0 = cur.blocks;
+			cur.members.push_back(m);
+			cur.blocks += items[m].blocks;
+			cur.cost += items[m].cost;
+			cur.shmem = std::max(cur.shmem, items[m].shmem);
+		}
+		plans.push_back(cur);
+	}
+	std::stable_sort(plans.begin(), plans.end(), [](Plan const& x, Plan const& y) { return x.cost > y.cost; });
+
+	// device memory: per launch, its descriptors then its owner table (each 256-byte aligned)
 	auto* bt = new Batch();
 	bt->device = di.id;
 	bt->nprob = static_cast<int>(items.size());
 	bt->bytes = bytes;
-	bt->variant.assign(items.size(), -1);
-	std::vector<BatchDesc> host(items.size());
-	std::vector<int64_t> tile0(items.size());
-	int64_t tile = 0;
-	for(size_t n = 0; n < items.size(); ++n) {
-		host[n]  = items[n].d;
-		tile0[n] = tile;
-		tile += items[n].tiles;
-		bt->order.push_back(items[n].index);
-		bt->variant[items[n].index] = items[n].d.variant;
+	bt->variant = variant;
+	std::size_t off = 0;
+	for(Plan const& p : plans) {
+		BatchLaunch L;
+		L.kernel = group_kernel(p.key);
+		L.blocks = p.blocks;
+		L.shmem = p.shmem;
+		L.desc_off = off;
+		off += (p.members.size() * sizeof(BatchDesc) + 255) / 256 * 256;
+		L.owner_off = off;
+		off += (static_cast<std::size_t>(p.blocks) * sizeof(uint32_t) + 255) / 256 * 256;
+		bt->launches.push_back(L);
+		bt->total_blocks += p.blocks;
 	}
-	bt->total_tiles = tile;
-	// every block draws once past the end: the counter must not wrap
-	if(tile >= (int64_t{1} << 32) - kBlock * 8192) {
-		delete bt;
-		return fail(TTVX_ERR_UNSUPPORTED, "ttvx_batch_create: too many tiles for one launch");
+	std::vector<unsigned char> host(off);
+	for(size_t l = 0; l < plans.size(); ++l) {
+		auto* descs = reinterpret_cast<BatchDesc*>(host.data() + bt->launches[l].desc_off);
+		auto* owner = reinterpret_cast<uint32_t*>(host.data() + bt->launches[l].owner_off);
+		for(size_t j = 0; j < plans[l].members.size(); ++j) {
+			Item const& it = items[plans[l].members[j]];
+			std::memcpy(&descs[j], &it.d, sizeof(BatchDesc));
+			std::fill(owner + it.d.block0, owner + it.d.block0 + it.blocks, static_cast<uint32_t>(j));
+		}
 	}
-	int resident = 0;
-	if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_batch, kBlock, 0) != hipSuccess || resident < 1) resident = 1;
-	bt->grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(tile, static_cast<int64_t>(di.cus) * resident)));
-	size_t const dbytes  = host.size() * sizeof(BatchDesc);
-	size_t const t0_off  = (dbytes + 255) / 256 * 256;
-	size_t const tbytes  = tile0.size() * sizeof(int64_t);
-	size_t const cnt_off = t0_off + (tbytes + 255) / 256 * 256;     // counter on its own 256-byte line
-	hipError_t e = hipMalloc(&bt->dmem, cnt_off + 256);
+	hipError_t e = hipMalloc(&bt->dmem, std::max<std::size_t>(off, 256));
 	if(e != hipSuccess) {
+		bt->dmem = nullptr;
 		delete bt;
 		return hip_fail(e, "hipMalloc (batch descriptors)");
 	}
-	bt->descs   = static_cast<BatchDesc*>(bt->dmem);
-	bt->tile0   = reinterpret_cast<int64_t*>(static_cast<char*>(bt->dmem) + t0_off);
-	bt->counter = reinterpret_cast<unsigned int*>(static_cast<char*>(bt->dmem) + cnt_off);
-	e = hipMemcpy(bt->descs, host.data(), dbytes, hipMemcpyHostToDevice);
-	if(e == hipSuccess) e = hipMemcpy(bt->tile0, tile0.data(), tbytes, hipMemcpyHostToDevice);
+	e = hipMemcpy(bt->dmem, host.data(), off, hipMemcpyHostToDevice);
 	if(e != hipSuccess) {
 		(void)hipFree(bt->dmem);
 		delete bt;
@@ -312,12 +386,15 @@ int ttvx_batch_launch(void* handle, int use_graph, void* stream)
 	clear_error();
 	auto* bt = static_cast<Batch*>(handle);
 	if(bt == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx_batch_launch: null handle");
+	int dev = -1;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return hip_fail(e, "hipGetDevice");
+	if(dev != bt->device) return fail(TTVX_ERR_ARGUMENT, "ttvx_batch_launch: the batch was created on another device");
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	if(!use_graph) return enqueue(*bt, s);
 	if(bt->graph_exec == nullptr) {
 		// capture once on a private stream (the legacy default stream cannot be captured); a failed
 		// earlier capture leaves the stream for this attempt and its graph is dropped
-		hipError_t e = hipSuccess;
 		if(bt->capture_stream == nullptr) {
 			e = hipStreamCreateWithFlags(&bt->capture_stream, hipStreamNonBlocking);
 			if(e != hipSuccess) {
@@ -341,7 +418,7 @@ int ttvx_batch_launch(void* handle, int use_graph, void* stream)
 			return hip_fail(e, "hipGraphInstantiate");
 		}
 	}
-	hipError_t e = hipGraphLaunch(bt->graph_exec, s);
+	e = hipGraphLaunch(bt->graph_exec, s);
 	return e == hipSuccess ? TTVX_OK : hip_fail(e, "hipGraphLaunch");
 }
 
@@ -350,9 +427,9 @@ int ttvx_batch_info(void* handle, int64_t* out4)
 	clear_error();
 	auto* bt = static_cast<Batch*>(handle);
 	if(bt == nullptr || out4 == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx_batch_info: null argument");
-	out4[0] = bt->total_tiles;
+	out4[0] = bt->total_blocks;
 	out4[1] = bt->bytes;
-	out4[2] = bt->grid;
+	out4[2] = static_cast<int64_t>(bt->launches.size());
 	out4[3] = bt->nprob;
 	return TTVX_OK;
 }

@@ -1,6 +1,7 @@
-// Diagnostic stream kernels used by the benchmark: a pure read stream (the practical HBM read
-// ceiling the product kernels are compared with) and a plain copy (a known-size kernel that
-// calibrates the FETCH_SIZE / WRITE_SIZE counters).
+// Diagnostic stream kernels used by the benchmark: read streams and read streams with a write
+// fraction (the practical HBM ceilings the product kernels are compared with, taken as the best of
+// two access patterns) and a plain copy (a known-size kernel that calibrates the FETCH_SIZE /
+// WRITE_SIZE counters).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -71,6 +72,35 @@ __global__ __launch_bounds__(kBlock) void k_mix_stream(const u32x4* __restrict__
 	}
 }
 
+// Compact sweep: block b reads the U*kBlock 16-byte vectors [b*U*kBlock, (b+1)*U*kBlock) of src,
+// all loads issued at once, writes its share dst[b*dchunk, (b+1)*dchunk) of the write stream and
+// exits; the dispatcher hands out blocks in address order, so the loads in flight form one compact
+// front moving through src (the pattern of the one-unit-per-wave product kernels).  Without a
+// write stream the sum goes to sink only when it equals a value it cannot take, which keeps the
+// loads alive.
+template<int U>
+__global__ __launch_bounds__(kBlock) void k_sweep_stream(const u32x4* __restrict__ src, int64_t n16,
+                                                         u32x4* __restrict__ dst, int64_t m16, int64_t dchunk,
+                                                         float* __restrict__ sink)
+{
+	int64_t const i0 = static_cast<int64_t>(blockIdx.x) * (U * kBlock) + threadIdx.x;
+	u32x4 v[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) {
+		int64_t const j = i0 + u * kBlock;
+		v[u] = __builtin_nontemporal_load(src + (j < n16 ? j : n16 - 1));
+	}
+	float s = 0.f;
+#pragma unroll
+	for(int u = 0; u < U; ++u) s += __uint_as_float(v[u].x) + __uint_as_float(v[u].y) + __uint_as_float(v[u].z) + __uint_as_float(v[u].w);
+	int64_t const d0 = static_cast<int64_t>(blockIdx.x) * dchunk;
+	int64_t const de = d0 + dchunk < m16 ? d0 + dchunk : m16;
+	u32x4 w;
+	w.x = w.y = w.z = w.w = __float_as_uint(s);
+	for(int64_t j = d0 + threadIdx.x; j < de; j += kBlock) __builtin_nontemporal_store(w, dst + j);
+	if(m16 == 0 && s == -1.0f) sink[0] = s;     // inputs are non-negative bit patterns of the benchmark
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_stream(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16)
 {
 	int64_t const stride = static_cast<int64_t>(gridDim.x) * kBlock;
@@ -122,6 +152,38 @@ int ttvx_bench_mix(const void* src, size_t src_bytes, void* dst, size_t dst_byte
 	                   static_cast<const u32x4*>(src), n16, chunk, static_cast<u32x4*>(dst), m16, dchunk);
 	hipError_t e = hipGetLastError();
 	return e == hipSuccess ? TTVX_OK : hip_fail(e, "mix stream launch");
+}
+
+int ttvx_bench_sweep(const void* src, size_t src_bytes, void* dst, size_t dst_bytes, int loads_per_thread,
+                     float* sink, void* stream)
+{
+	using namespace ttvx;
+	clear_error();
+	DeviceInfo di;
+	if(!current_device(di)) return TTVX_ERR_DEVICE;
+	if(src_bytes % 16 != 0 || dst_bytes % 16 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0 ||
+	   reinterpret_cast<uintptr_t>(dst) % 16 != 0 || src_bytes == 0 || sink == nullptr)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_sweep: 16-byte aligned buffers and sizes and a sink required");
+	if(loads_per_thread != 2 && loads_per_thread != 4 && loads_per_thread != 8)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_sweep: loads_per_thread must be 2, 4 or 8");
+	int rc = check_device_span(src, src_bytes, "src");
+	if(rc == TTVX_OK && dst_bytes > 0) rc = check_device_span(dst, dst_bytes, "dst");
+	if(rc == TTVX_OK) rc = check_device_span(sink, 4, "sink");
+	if(rc != TTVX_OK) return rc;
+	int64_t const n16 = static_cast<int64_t>(src_bytes / 16), m16 = static_cast<int64_t>(dst_bytes / 16);
+	int64_t const per = static_cast<int64_t>(loads_per_thread) * kBlock;
+	int64_t const grid = (n16 + per - 1) / per;
+	if(grid > kMaxLeanBlocks) return fail(TTVX_ERR_ARGUMENT, "ttvx_bench_sweep: source too large for one launch");
+	int64_t const dchunk = (m16 + grid - 1) / grid;
+	auto* s = static_cast<const u32x4*>(src);
+	auto* d = static_cast<u32x4*>(dst);
+	hipStream_t st = static_cast<hipStream_t>(stream);
+	dim3 const g(static_cast<unsigned>(grid));
+	if(loads_per_thread == 2) hipLaunchKernelGGL(k_sweep_stream<2>, g, dim3(kBlock), 0, st, s, n16, d, m16, dchunk, sink);
+	else if(loads_per_thread == 4) hipLaunchKernelGGL(k_sweep_stream<4>, g, dim3(kBlock), 0, st, s, n16, d, m16, dchunk, sink);
+	else hipLaunchKernelGGL(k_sweep_stream<8>, g, dim3(kBlock), 0, st, s, n16, d, m16, dchunk, sink);
+	hipError_t e = hipGetLastError();
+	return e == hipSuccess ? TTVX_OK : hip_fail(e, "sweep stream launch");
 }
 
 int ttvx_bench_copy(const void* src, void* dst, size_t bytes, void* stream)

@@ -164,7 +164,8 @@ std::size_t acc_size(const DType& dt) { return dt.code == TTVX_F64 ? 8 : 4; }
 //   TTVX_CONTIG_DEFAULT       work assignment of k_dot / k_col: 0 = grid-stride over up to
 //                             TTVX_GRID_PER_CU blocks per CU (faster for rows and narrow column
 //                             tiles), 1 = one contiguous range per block
-//   TTVX_GRID_PER_CU          block cap per CU of the column and strided launches
+//   TTVX_GRID_PER_CU          block cap per CU of the col_u4, col_wide, col_split, col_k_lanes and
+//                             strided launches
 //   TTVX_COL_WIDE_U           slab units per lane of the col_wide variant
 //   TTVX_DOT_GROUPS_PER_WAVE  row groups per wave the looping row launches aim at (twice that for
 //                             1- and 2-byte types, whose groups hold more rows)
@@ -189,8 +190,8 @@ constexpr int kGridPerCU = TTVX_GRID_PER_CU;
 #endif
 constexpr int kDotGroupsPerWave = TTVX_DOT_GROUPS_PER_WAVE;
 constexpr int kDotGridPerCU = 64;
-// largest grid of k_rows and k_slab (one work unit per wave or block)
-constexpr int64_t kMaxGrid = int64_t{1} << 30;
+// largest grid of k_rows and k_slab, which take one work unit per wave or block and so need a
+// block for every unit: a launch holds fewer than 2^32 work-items
 // col_narrow's row chunk in lanes (16 lanes x 16-byte vectors = 256 bytes: 6.61 -> 6.81 TB/s on
 // FP32 slabs of 128 x 128 against whole 512-byte rows)
 #ifndef TTVX_NARROW_LANES
@@ -253,11 +254,11 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		P.kiter  = static_cast<int>(std::min<int64_t>(1 << 30, ceil_div(std::min(kchunk, f.K), kstride)));
 		P.kpart  = f.K % cfg.vec != 0 ? 1 : 0;
 		cfg.ksplit = static_cast<int>(ksplit);
-		if(P.kiter == 1 && ksplit == 1) {
+		if(P.kiter == 1 && ksplit == 1 && ceil_div(P.groups, kBlock / 64) <= kMaxLeanBlocks) {
 			// rows of one vector per lane: k_rows, one group per wave and no cap on the grid (short-lived
 			// blocks handed out in address order sweep A as one compact front)
 			cfg.lean = LEAN_ROWS;
-			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(P.groups, kBlock / 64), kMaxGrid)));
+			cfg.grid = static_cast<int>(std::max<int64_t>(1, ceil_div(P.groups, kBlock / 64)));
 		} else {
 			// about kDotGroupsPerWave row groups per wave (twice that for the narrow types, whose groups
 			// hold more rows): enough waves to fill the GPU, and each wave still has a next group to load
@@ -316,7 +317,8 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 			P.kchunk = f.K;
 			cfg.ksplit = 1;
 			cfg.lean = LEAN_SLAB;
-			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(P.items, kMaxGrid)));
+			cfg.grid = static_cast<int>(std::max<int64_t>(1, P.items));
+			if(P.items > kMaxLeanBlocks) return false;               // one block per item
 			if(fp64_final && dt.code != TTVX_F64) return false;      // no split partials in k_slab
 			return true;
 		}
@@ -331,7 +333,11 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		int64_t const kchunk = ceil_div(f.K, ksplit);
 		P.kchunk = kchunk;
 		cfg.ksplit = static_cast<int>(ceil_div(f.K, kchunk));
-		cfg.grid   = static_cast<int>(std::max<int64_t>(1, std::min(P.items, gridcap)));
+		// one- and two-unit tiles: one item per block on an uncapped grid (short-lived blocks handed
+		// out in address order; MI355X, FP32: +1-2 % on 128^4 with I = 128, +4-7 % on 16^7 with
+		// I = 16..256 against blocks looping over items); wider tiles keep the capped grid
+		int64_t const cap = (v == V_COL_U1 || v == V_COL_U2) ? kMaxLeanBlocks : gridcap;
+		cfg.grid   = static_cast<int>(std::max<int64_t>(1, std::min(P.items, cap)));
 	} else if(v == V_COL_K_LANES) {
 		if(!dot_shape) return false;
 		// the column kernel on a row problem: lanes along k (scalar loads), summed across 64 lanes

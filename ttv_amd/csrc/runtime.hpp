@@ -58,6 +58,9 @@ DType dtype_of(int code);
 // k_slab for col_narrow; LEAN_NONE for the looping k_dot / k_col / k_strided.
 enum Lean { LEAN_NONE = 0, LEAN_ROWS = 1, LEAN_SLAB = 2 };
 
+// Most blocks of kBlock threads in one launch (fewer than 2^32 work-items per grid dimension).
+constexpr int64_t kMaxLeanBlocks = (int64_t{1} << 32) / kBlock - 1;
+
 struct Config {
 	Kind kind = KIND_STRIDED;
 	Lean lean = LEAN_NONE;
