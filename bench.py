@@ -131,7 +131,7 @@ def summarize_profiles(d, traffic_out=None, kind="f32"):
             return list(csv.DictReader(f))
 
     def is_product(name):
-        return any(k in name for k in ("k_dot", "k_col", "k_strided"))
+        return any(k in name for k in ("k_rows", "k_dot", "k_slab", "k_col", "k_strided"))
 
     out = []
     stats = rows(os.path.join(d, "prof_trace", "run_kernel_stats.csv"))
