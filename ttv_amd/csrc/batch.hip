@@ -260,8 +260,7 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 		Config cfg;
 		if(v == V_STRIDED || !configure(v, f, dt, aaddr, di.cus, 0u, cfg) || cfg.ksplit != 1 || cfg.args.part64)
 			return fail(TTVX_ERR_UNSUPPORTED, where + "the batched kernel needs A and C to fold to three dense-inner dimensions");
-		Item it;
-		std::memset(&it, 0, sizeof(it));
+		Item it{};
 		int body, lprc = 0;
 		if(cfg.lean == LEAN_SLAB) {
 			body = BB_SLAB;

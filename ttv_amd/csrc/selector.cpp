@@ -30,8 +30,10 @@ bool g_weights_decided = false;     // set, cleared or default load attempted
 using TKey = std::tuple<int, int64_t, int64_t, int64_t>;
 std::map<TKey, int> g_table;
 bool g_table_decided = false;
-// decision cache: (dtype, O, K, I, alignment class, qpos, p) -> variant
-using Key = std::tuple<int, int64_t, int64_t, int64_t, int, int, int>;
+// decision cache of folded 3-D problems: (dtype, O, K, I, alignment class, qpos, p, stride class)
+// -> variant; the stride class (unit k stride, unit i strides of A and C) is what decides which
+// variants can run a folded problem besides its extents
+using Key = std::tuple<int, int64_t, int64_t, int64_t, int, int, int, int>;
 std::map<Key, int> g_cache;
 } // namespace
 
@@ -173,9 +175,12 @@ int set_selector_table(const int64_t* rows, std::size_t count)
 
 int learned_variant(const Fold& f, const DType& dt, uintptr_t a_addr, unsigned flags)
 {
+	// problems that do not fold run the strided kernel whatever the network says
+	if(!f.three_d) return -1;
 	std::shared_ptr<const std::vector<float>> w;
 	int const align = static_cast<int>(a_addr % 16);
-	Key const key{dt.code, f.O, f.K, f.I, align, f.qpos, f.p};
+	int const strides = (f.sAk == 1 ? 1 : 0) | (f.sAi == 1 ? 2 : 0) | (f.sCi == 1 ? 4 : 0);
+	Key const key{dt.code, f.O, f.K, f.I, align, f.qpos, f.p, strides};
 	{
 		std::lock_guard<std::mutex> lock(g_sel_mutex);
 		auto it = g_cache.find(key);
