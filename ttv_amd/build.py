@@ -17,7 +17,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = [
     "plan.cpp", "selector.cpp", "runtime.hip", "api.hip", "quant.hip",
-    "launch_f32.hip", "launch_f64.hip", "launch_bf16.hip", "launch_fp8.hip",
+    "launch_f32.hip", "launch_f64.hip", "launch_bf16.hip", "launch_fp8.hip", "launch_bf16_sub.hip",
+    "launch_fp8_sub.hip",
 ]
 OPTIONAL = ["batch.hip", "mlp.hip", "bench_util.hip"]
 

@@ -107,6 +107,9 @@ int launch_f32 (const Config& cfg, const void* a, const void* b, void* c, bool k
 int launch_f64 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
 int launch_bf16(const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
 int launch_fp8 (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
+// BF16 / FP8 kernels with realigned loads, for tensors whose vectors do not all start on 4-byte boundaries
+int launch_bf16_sub(const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
+int launch_fp8_sub (const Config& cfg, const void* a, const void* b, void* c, bool kahan, hipStream_t s);
 
 // Device memory a caller provides for the split partial sums instead of the library's buffers.
 struct Scratch {
