@@ -226,7 +226,9 @@ int ttvx_mlp_adam(float* params, const float* grad, float* m1, float* m2, int st
 
 /* ---- batched products (no counterpart in the reference: it runs one product per call) ---- */
 /* One FP32 problem of a batch; C's extents and layout are derived from A as in the reference
- * (shape.h:103-158, layout.h:143-207); wc = NULL means C's dense strides in that layout. */
+ * (shape.h:103-158, layout.h:143-207); wc = NULL means C's dense strides in that layout.
+ * variant < 0 takes the library's selection; variant >= 0 fixes the kernel variant of this problem
+ * (the per-product counterpart of the reference's policy tags, tags.h:21-63). */
 typedef struct ttvx_batch_problem {
 	size_t q, p;
 	const float* a;
@@ -236,11 +238,13 @@ typedef struct ttvx_batch_problem {
 	const float* b;
 	float* c;
 	const size_t* wc;
+	int variant;
 } ttvx_batch_problem;
 
 /* Validates and plans `count` problems (device pointers, fixed for the batch's lifetime) and
- * returns an opaque handle.  Each problem keeps the tile configuration of its own selected
- * variant; contractions longer than 2^20 and problems that fold to the strided kernel are
+ * returns an opaque handle.  Each problem keeps the tile configuration of its variant (its own, or
+ * the one a single call selects, split variants in their unsplit form); contractions longer than
+ * 2^20, problems that fold to the strided kernel and variants that cannot run their problem are
  * rejected (TTVX_ERR_UNSUPPORTED). */
 int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** batch);
 /* Computes every problem of the batch on `stream`: one launch per group of problems that share a
@@ -250,9 +254,9 @@ int ttvx_batch_launch(void* batch, int use_graph, void* stream);
 /* out4 = {blocks over all launches, algorithmic bytes, launches, problems}. */
 int ttvx_batch_info(void* batch, int64_t* out4);
 /* Kernel variant (0 .. ttvx_num_variants()-1) whose tiles run problem `index` of the batch: the
- * variant a single call selects, with the split variants replaced by their unsplit forms
- * (col_split -> col_u1, dot_split -> dot_u2).  A single call forced to it gives bit-identical
- * outputs.  -1 on error. */
+ * problem's own variant, or the one a single call selects, split variants replaced by their
+ * unsplit forms (col_split -> col_u1, dot_split -> dot_u2).  A single call forced to it gives
+ * bit-identical outputs.  -1 on error. */
 int ttvx_batch_variant(void* batch, size_t index);
 int ttvx_batch_destroy(void* batch);
 

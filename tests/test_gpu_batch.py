@@ -177,6 +177,40 @@ def test_batch_every_variant(ttvx, device):
             "col_wide"} <= ran, ran
 
 
+def test_batch_per_problem_variants(ttvx, device):
+    """Variants fixed per problem (the counterpart of the reference's per-call policy tags): each
+    problem runs the one it names (split variants in their unsplit form), bit-identical to a single
+    launch forced to it; a variant that cannot run its problem is refused with the problem's index."""
+    import torch
+    from ttv_amd.batch import Batch
+    names = ttvx.variant_names()
+    specs = [_spec([300, 40, 9], 2, [1, 2, 3]), _spec([64, 9, 70], 1, [1, 2, 3]), _spec([1000, 7, 3], 1, [1, 2, 3]),
+             _spec([17, 33, 21], 3, [3, 1, 2]), _spec([256, 64, 8], 2, [1, 2, 3])]
+    probs = _make(torch, device, specs, 12)
+    want = []
+    for q, A, b in probs:
+        na, wa = list(A.shape), list(A.stride())
+        ok = [v for v, n in enumerate(names) if n != "strided" and
+              ttvx.variant_valid(q, na, wa, ttvx.layout_from_strides(na, wa), v)]
+        want.append(ok[len(want) % len(ok)])
+    bt = Batch(probs, variants=want)
+    got = [bt.variant(n) for n in range(len(probs))]
+    unsplit = {"col_split": "col_u1", "dot_split": "dot_u2"}
+    assert got == [names.index(unsplit.get(names[v], names[v])) for v in want], (got, want)
+    bt.launch(graph=False)
+    torch.cuda.synchronize()
+    batched = [_flat(C).cpu().numpy().copy() for C in bt.outputs]
+    _check_against_single(ttvx, bt, probs, batched)
+    for n, (q, A, b) in enumerate(probs):
+        _check_oracle(q, A, b, bt.outputs[n], batched[n])
+    bt.close()
+    col = probs[0]           # a column problem: no row variant can run it
+    with pytest.raises(ttvx.TTVError, match="problem 1: variant dot_u2 cannot run"):
+        Batch([probs[1], col], variants=[None, names.index("dot_u2")])
+    with pytest.raises(ttvx.TTVError, match="problem 0: no kernel variant"):
+        Batch([col], variants=[len(names)])
+
+
 def test_batch_graph_replays(ttvx, device):
     import torch
     from ttv_amd.batch import Batch

@@ -75,7 +75,8 @@ _SIGS = {
 class BatchProblem(ctypes.Structure):
     """ttvx_batch_problem (ttvx.h)."""
     _fields_ = [("q", ctypes.c_size_t), ("p", ctypes.c_size_t), ("a", ctypes.c_void_p), ("na", _szp),
-                ("wa", _szp), ("pia", _szp), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p), ("wc", _szp)]
+                ("wa", _szp), ("pia", _szp), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p), ("wc", _szp),
+                ("variant", ctypes.c_int)]
 
 _lib = None
 

@@ -12,7 +12,9 @@ from .meta import generate_output_layout, generate_output_shape, generate_stride
 
 
 class Batch:
-    def __init__(self, problems, outputs=None):
+    def __init__(self, problems, outputs=None, variants=None):
+        """problems: (q, A, b) triples; outputs: C tensors to write (allocated when None);
+        variants: per-problem kernel variant (None or -1 entries: the library's selection)."""
         import torch
         self._keep = []
         arr = (BatchProblem * len(problems))()
@@ -32,7 +34,8 @@ class Batch:
             self.outputs.append(C)
             sa, sw, sp, sc = sizes(na), sizes(wa), sizes(pia), sizes(wc)
             self._keep += [A, b, C, sa, sw, sp, sc]
-            arr[n] = BatchProblem(q, len(na), A.data_ptr(), sa, sw, sp, b.data_ptr(), C.data_ptr(), sc)
+            v = -1 if variants is None or variants[n] is None else int(variants[n])
+            arr[n] = BatchProblem(q, len(na), A.data_ptr(), sa, sw, sp, b.data_ptr(), C.data_ptr(), sc, v)
         self._arr = arr
         h = ctypes.c_void_p()
         check(lib().ttvx_batch_create(len(problems), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(h)))

@@ -253,13 +253,18 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 			return fail(TTVX_ERR_UNSUPPORTED, where + "the batched kernel does not split contractions longer than 2^20; "
 			                                          "run such products alone (they use an FP64 final reduction)");
 		uintptr_t const aaddr = reinterpret_cast<uintptr_t>(pr.a);
-		// the variant a single call would take, with the contraction never split
-		int v = select_variant(f, dt, aaddr, 0u);
+		// the problem's own variant, or the one a single call would take; the contraction is never split
+		if(pr.variant >= ttvx_num_variants())
+			return fail(TTVX_ERR_ARGUMENT, where + "no kernel variant " + std::to_string(pr.variant));
+		int v = pr.variant >= 0 ? pr.variant : select_variant(f, dt, aaddr, 0u);
 		if(v == V_COL_SPLIT) v = V_COL_U1;
 		if(v == V_DOT_SPLIT) v = V_DOT_U2;
 		Config cfg;
 		if(v == V_STRIDED || !configure(v, f, dt, aaddr, di.cus, 0u, cfg) || cfg.ksplit != 1 || cfg.args.part64)
-			return fail(TTVX_ERR_UNSUPPORTED, where + "the batched kernel needs A and C to fold to three dense-inner dimensions");
+			return fail(TTVX_ERR_UNSUPPORTED, where + (pr.variant >= 0 ? std::string("variant ") + ttvx_variant_name(pr.variant) +
+			                                                            " cannot run this problem in a batch"
+			                                                          : std::string("the batched kernel needs A and C to fold "
+			                                                                        "to three dense-inner dimensions")));
 		Item it{};
 		int body, lprc = 0;
 		if(cfg.lean == LEAN_SLAB) {
