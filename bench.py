@@ -103,16 +103,60 @@ FP8_FIBERS_PER_SCALE = 32                        # ttv_amd.default_fibers_per_sc
 FP8_SCALES = (N // 128) // FP8_FIBERS_PER_SCALE  # one float per 4096 elements
 
 
+ORDER8_SHAPE = [4096, 4, 8, 4, 8, 4, 2, 64]
+ORDER8_N = 1 << 31
+
+
 def profile_cases(kind):
     """(label, algorithmic bytes per launch) of the products `bench.py --profile KIND` dispatches,
     in dispatch order (each 3 times)."""
     if kind == "f32":
         return [("%s q=%d" % (lname, q), case_bytes(128)) for lname, q in CASE_ORDER]
+    if kind == "other":
+        out = [("16^7 %s q=%d" % (lname, q), N * 4 + (N // 16) * 4 + 16 * 4)
+               for lname in ("first", "last") for q in range(1, 8)]
+        n3 = 1 << 24
+        out.append(("fp64 256^3 q=2", n3 * 8 + 65536 * 8 + 256 * 8))
+        for q in (1, 4, 8):
+            nq = ORDER8_SHAPE[q - 1]
+            out.append(("order-8 q=%d" % q, ORDER8_N * 4 + (ORDER8_N // nq) * 4 + nq * 4))
+        return out
     out = []
     for lname, q in CASE_ORDER:
         out.append(("bf16 %s q=%d" % (lname, q), case_bytes(128, elem=2)))
         out.append(("fp8 %s q=%d" % (lname, q), case_bytes(128, elem=1, extra=FP8_SCALES * 4)))
     return out
+
+
+def other_profile_runs(torch, ttv_amd, Aflat):
+    """(label, launch) of the `--profile other` products, operands and outputs allocated up front,
+    in profile_cases("other") order: FP32 16^7 (every q, first/last-order, on the 128^4 data), FP64
+    256^3 q = 2, FP32 order-8 (the benchmark's seeded layout) q = 1, 4, 8."""
+    import numpy as np
+    from ttv_amd.meta import generate_strides
+    dev = Aflat.device
+    runs = []
+    sh = [16] * 7
+    for lname, pia in (("first", list(range(1, 8))), ("last", list(range(7, 0, -1)))):
+        A = Aflat.as_strided(sh, generate_strides(sh, pia))
+        for q in range(1, 8):
+            b = torch.rand(16, device=dev)
+            C = ttv_amd.ttv(q, A, b)
+            runs.append(("16^7 %s q=%d" % (lname, q), lambda q=q, A=A, b=b, C=C: ttv_amd.ttv(q, A, b, out=C)))
+    n3 = 1 << 24
+    A64 = Aflat[:n3].double().view(256, 256, 256).permute(2, 1, 0)
+    b64 = torch.rand(256, device=dev, dtype=torch.float64)
+    C64 = ttv_amd.ttv(2, A64, b64)
+    runs.append(("fp64 256^3 q=2", lambda: ttv_amd.ttv(2, A64, b64, out=C64)))
+    pia8 = [int(x) for x in np.random.default_rng(31).permutation(8) + 1]
+    A8 = torch.empty(ORDER8_N, dtype=torch.float32, device=dev).uniform_(-1, 1).as_strided(
+        ORDER8_SHAPE, generate_strides(ORDER8_SHAPE, pia8))
+    for q in (1, 4, 8):
+        b = torch.rand(ORDER8_SHAPE[q - 1], device=dev)
+        C = ttv_amd.ttv(q, A8, b)
+        runs.append(("order-8 q=%d" % q, lambda q=q, b=b, C=C: ttv_amd.ttv(q, A8, b, out=C)))
+    torch.cuda.synchronize()
+    return runs
 
 
 def summarize_profiles(d, traffic_out=None, kind="f32"):
@@ -187,8 +231,11 @@ def summarize_profiles(d, traffic_out=None, kind="f32"):
         out.append("  %-15s %-38s %7.1f us  fetch %.4e B  write %.4e B  (%.4f x algorithmic %d B, %.1f GB/s)" % (
             label, name.replace("ttvx::", ""), us, fb, wb, (fb + wb) / alg, alg, alg / (us * 1e-6) / 1e9))
     if traffic_out:
-        workload = ("FP32 128^4, q=1..4 x {first,last}-order" if kind == "f32" else
-                    "128^4 with A in BF16 and FP8-e4m3 (one scale per 4096 elements), q=1..4 x {first,last}-order")
+        workload = {"f32": "FP32 128^4, q=1..4 x {first,last}-order",
+                    "precision": "128^4 with A in BF16 and FP8-e4m3 (one scale per 4096 elements), q=1..4 x "
+                                 "{first,last}-order",
+                    "other": "FP32 16^7 (q=1..7 x {first,last}-order), FP64 256^3 q=2, FP32 order-8 "
+                             "(4096,4,8,4,8,4,2,64) q=1,4,8"}[kind]
         with open(traffic_out, "w") as f:
             json.dump(dict(workload=workload, fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), cases=cases),
                       f, indent=1)
@@ -206,12 +253,14 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the 16^7 FP32 and 256^3 FP64 cases")
     ap.add_argument("--no-order8", action="store_true", help="skip the order-8 precision configuration")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--profile", choices=("f32", "precision"),
+    ap.add_argument("--profile", choices=("f32", "precision", "other"),
                     help="short fixed dispatch sequence for rocprofv3: copy and read streams, then each of the "
-                         "8 FP32 products (f32) or the 8 BF16 and 8 FP8 products (precision) 3 times; prints no JSON")
+                         "8 FP32 products (f32), the 8 BF16 and 8 FP8 products (precision), or the FP32 16^7, "
+                         "FP64 256^3 and FP32 order-8 products (other) 3 times; prints no JSON")
     ap.add_argument("--summarize", metavar="DIR",
                     help="print the summary of the rocprofv3 runs of --profile under DIR (no GPU needed)")
-    ap.add_argument("--kind", choices=("f32", "precision"), default="f32", help="with --summarize: the --profile kind")
+    ap.add_argument("--kind", choices=("f32", "precision", "other"), default="f32",
+                    help="with --summarize: the --profile kind")
     ap.add_argument("--traffic-out", metavar="JSON", help="with --summarize: write the per-case traffic here")
     args = ap.parse_args()
     if args.summarize:
@@ -263,6 +312,7 @@ def main():
                 Aq = Abf.as_strided(SHAPE, wa)
                 prep.append((c, Aq, codes, scales, fps, ttv_amd.ttv_bf16(q, Aq, c["b"]),
                              ttv_amd.ttv_fp8(q, codes, scales, fps, c["b"])))
+        other = other_profile_runs(torch, ttv_amd, Aflat) if args.profile == "other" else []
         dst = torch.empty_like(Aflat)
         partial = torch.empty(256 * 32, dtype=torch.float32, device=dev)
         torch.cuda.synchronize()
@@ -276,6 +326,11 @@ def main():
                 log("profile case", c["layout"], c["q"], c["kernel"])
                 for _ in range(3):
                     run_case(c)
+        elif args.profile == "other":
+            for label, run in other:
+                log("profile case", label)
+                for _ in range(3):
+                    run()
         else:
             for c, Aq, codes, scales, fps, Cb, C8 in prep:
                 log("profile case bf16, fp8", c["layout"], c["q"])
@@ -475,10 +530,23 @@ def order8_leg(torch, np, ttv_amd, log, reps=5):
         for k in ("bf16", "fp8"):
             case[k]["speedup"] = round(case["fp32"]["ms"] / case[k]["ms"], 3)
             case[k]["ideal_speedup"] = round(case["fp32"]["bytes"] / case[k]["bytes"], 3)
+        # the FP32 product's ceiling: the compact-sweep stream reading the same 8 GiB and writing the
+        # product's output bytes
+        wbuf = torch.empty(outc, dtype=torch.float32, device=dev)
+        L = ttv_amd.lib()
+        fsink = torch.zeros(1, dtype=torch.float32, device=dev)
+        st = torch.cuda.current_stream().cuda_stream
+        cms = min(timed_cycle(torch, ttv_amd, [lambda u=u: ttv_amd.check(L.ttvx_bench_sweep(
+            A.data_ptr(), n * 4, wbuf.data_ptr(), outc * 4, u, fsink.data_ptr(), st))], reps, sink,
+            us_per_launch=2000.0) for u in (2, 4, 8))
+        del wbuf
+        case["fp32"]["ceiling_gbs"] = round((n * 4 + outc * 4) / (cms * 1e-3) / 1e9, 1)
+        case["fp32"]["of_ceiling"] = round(case["fp32"]["gbs"] / case["fp32"]["ceiling_gbs"], 4)
         case["max_abs_C"] = rmax
         res["cases"].append(case)
-        log("  order-8 q=%d (n_q=%d): fp32 %.3f ms %.0f GB/s | bf16 x%.2f (ideal %.2f) err %.2e | fp8 x%.2f (ideal %.2f) "
-            "err %.2e" % (q, nq, case["fp32"]["ms"], case["fp32"]["gbs"], case["bf16"]["speedup"],
+        log("  order-8 q=%d (n_q=%d): fp32 %.3f ms %.0f GB/s (%.3f of ceiling) | bf16 x%.2f (ideal %.2f) err %.2e | "
+            "fp8 x%.2f (ideal %.2f) err %.2e" % (q, nq, case["fp32"]["ms"], case["fp32"]["gbs"], case["fp32"]["of_ceiling"],
+                                                 case["bf16"]["speedup"],
                           case["bf16"]["ideal_speedup"], case["bf16"]["rel_l2"], case["fp8"]["speedup"],
                           case["fp8"]["ideal_speedup"], case["fp8"]["rel_l2"]))
         del codes, scales, C32, C16, C8, ref
@@ -540,9 +608,20 @@ def other_configs_leg(torch, np, ttv_amd, Aflat, mixed, log, reps=10):
     ms = timed_cycle(torch, ttv_amd, runs, reps * len(runs), sink, us_per_launch=100.0)
     gbs = nbytes / (ms * 1e-3) / 1e9
     info = ttv_amd.plan_info(2, [256] * 3, list(A64.stride()), [1, 2, 3], ttv_amd.F64)
+    # its ceiling: the compact-sweep read stream over 128 MiB with a 1/256 write stream (the FP64
+    # product's output fraction), cycling over the same 8 copies
+    L = ttv_amd.lib()
+    fsink = torch.zeros(1, dtype=torch.float32, device=dev)
+    wbuf = torch.empty(n3 * 8 // 256 // 4, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    cms = min(timed_cycle(torch, ttv_amd, [(lambda T=T, u=u: ttv_amd.check(L.ttvx_bench_sweep(
+        T.data_ptr(), n3 * 8, wbuf.data_ptr(), wbuf.numel() * 4, u, fsink.data_ptr(), stream.cuda_stream)))
+        for T in A64s], reps * len(A64s), sink, us_per_launch=100.0) for u in (2, 4, 8))
+    cgbs = (n3 * 8 + wbuf.numel() * 4) / (cms * 1e-3) / 1e9
     res["fp64_256e3"] = dict(q=2, layout="first", kernel=ttv_amd.variant_names()[info["variant"]], ms=round(ms, 4),
-                             gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4), copies=len(A64s))
-    log("  256^3 FP64 q=2 %.4f ms %7.1f GB/s" % (ms, gbs))
+                             gbs=round(gbs, 1), frac=round(gbs * 1e9 / PEAK_BPS, 4), copies=len(A64s),
+                             ceiling_128mib_gbs=round(cgbs, 1), of_ceiling=round(gbs / cgbs, 4))
+    log("  256^3 FP64 q=2 %.4f ms %7.1f GB/s (%.3f of the 128 MiB read ceiling, %.1f GB/s)" % (ms, gbs, gbs / cgbs, cgbs))
     fr = [c["frac"] for c in res["fp32_16e7"]]
     res["fp32_16e7_min_frac"] = min(fr)
     res["fp32_16e7_mean_frac"] = round(float(np.mean(fr)), 4)
