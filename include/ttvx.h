@@ -260,6 +260,16 @@ int ttvx_batch_info(void* batch, int64_t* out4);
 int ttvx_batch_variant(void* batch, size_t index);
 int ttvx_batch_destroy(void* batch);
 
+/* ---- fused multi-vector contraction (ttvs) ------------------------------------------------ */
+/* out[i] = prod_m vectors[m][i_m] with i = i_0 + n_0*(i_1 + n_1*(...)) (vector 0 fastest), on the
+ * device, stream-ordered; 1..32 device vectors.  The weights of the fused form of ttvpy's ttvs
+ * (ttvpy/src/wrapped_ttv.cpp:141-194): with A dense, every mode but q contracts as one row product
+ * with the Kronecker product of the vectors of the modes stored before q and one column product
+ * with that of the modes stored after q (no counterpart in the reference, which chains p - 1
+ * products). */
+int ttvx_kron_f32(const float* const* vectors, const size_t* lengths, size_t count, float* out, void* stream);
+int ttvx_kron_f64(const double* const* vectors, const size_t* lengths, size_t count, double* out, void* stream);
+
 /* ---- diagnostics (no counterpart in the reference; used by the benchmark) -------------- */
 /* Pure read stream over `bytes` (16-byte aligned), each block one contiguous chunk, writing one
  * float per block to `partial` (blocks = min(partial_count, 64 per CU)). */

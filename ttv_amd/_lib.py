@@ -58,6 +58,8 @@ _SIGS = {
     "ttvx_bench_copy": ([_vp, _vp, _sz, _vp], _int),
     "ttvx_bench_mix": ([_vp, _sz, _vp, _sz, _vp], _int),
     "ttvx_bench_sweep": ([_vp, _sz, _vp, _sz, _int, _vp, _vp], _int),
+    "ttvx_kron_f32": ([_vp, _vp, _sz, _vp, _vp], _int),
+    "ttvx_kron_f64": ([_vp, _vp, _sz, _vp, _vp], _int),
     "ttvx_bench_spin": ([_int, _vp, _vp], _int),
     "ttvx_mlp_work_floats": ([_int], _sz),
     "ttvx_mlp_forward": ([_vp, _vp, _int, _vp, _vp, _vp], _int),

@@ -20,7 +20,7 @@ SOURCES = [
     "launch_f32.hip", "launch_f64.hip", "launch_bf16.hip", "launch_fp8.hip", "launch_bf16_sub.hip",
     "launch_fp8_sub.hip",
 ]
-OPTIONAL = ["batch.hip", "mlp.hip", "bench_util.hip"]
+OPTIONAL = ["batch.hip", "mlp.hip", "bench_util.hip", "chain.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
