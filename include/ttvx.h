@@ -270,6 +270,15 @@ int ttvx_batch_destroy(void* batch);
 int ttvx_kron_f32(const float* const* vectors, const size_t* lengths, size_t count, float* out, void* stream);
 int ttvx_kron_f64(const double* const* vectors, const size_t* lengths, size_t count, double* out, void* stream);
 
+/* ---- fault injection (no counterpart in the reference; a test hook) --------------------- */
+#define TTVX_FAULT_ALLOC  0  /* device allocations: staging, split partials, batches, ttvx_device_alloc */
+#define TTVX_FAULT_COPY   1  /* host <-> device copies of the synchronous entry points */
+#define TTVX_FAULT_LAUNCH 2  /* product launches (the kernel is not enqueued) */
+/* The next `count` operations at `site` fail as if HIP had reported out of memory, an invalid copy
+ * or a launch failure, without touching the device (count 0 disarms): the library's error paths
+ * (status, message, state left for the next call) can be exercised on a healthy GPU.  Process-wide. */
+int ttvx_debug_fail_next(int site, int count);
+
 /* ---- diagnostics (no counterpart in the reference; used by the benchmark) -------------- */
 /* Pure read stream over `bytes` (16-byte aligned), each block one contiguous chunk, writing one
  * float per block to `partial` (blocks = min(partial_count, 64 per CU)). */

@@ -353,9 +353,34 @@ static void examples()
 	}
 }
 
+// Device errors forced with the fault-injection hook: the drop-in interfaces throw
+// std::runtime_error with the library's message, and the next product works.
+static void injected_failures()
+{
+	auto A = tensor<float>({4, 3, 2});
+	auto B = tensor<float>({3, 1});
+	std::iota(A.begin(), A.end(), 1);
+	std::fill(B.begin(), B.end(), 1);
+	std::vector<float> const expect = {15, 18, 21, 24, 51, 54, 57, 60};
+	for(int site : {TTVX_FAULT_ALLOC, TTVX_FAULT_COPY, TTVX_FAULT_LAUNCH}) {
+		ttvx_debug_fail_next(site, 1);
+		std::string msg;
+		try {
+			auto C = A(2) * B;
+			(void)C;
+		} catch(std::runtime_error const& e) {
+			msg = e.what();
+		}
+		CHECK(msg.find("HIP error") != std::string::npos, "injected failure %d: '%s'", site, msg.c_str());
+		auto C = A(2) * B;
+		CHECK(C.data() == expect, "product after injected failure %d", site);
+	}
+}
+
 int main()
 {
 	examples();
+	injected_failures();
 	mtv_tests();
 	device_tensors();
 	all_policies<double>();

@@ -53,3 +53,18 @@ def test_reduced_precision_entry_points_check_operands():
         ttv_amd.ttv_bf16(2, torch.zeros((2, 3, 4)), b)
     with pytest.raises(ttv_amd.TTVError, match="uint8 GPU tensor"):
         ttv_amd.ttv_fp8(2, a16, torch.ones(1), 8, b)
+
+
+def test_fault_hook_arguments():
+    """ttvx_debug_fail_next validates its site and count without a GPU; arming and disarming a
+    valid site succeeds."""
+    import ttv_amd
+    from ttv_amd import _lib
+    L = ttv_amd.lib()
+    assert L.ttvx_debug_fail_next(3, 1) == _lib.ERR_ARGUMENT
+    assert L.ttvx_debug_fail_next(-1, 1) == _lib.ERR_ARGUMENT
+    assert L.ttvx_debug_fail_next(_lib.FAULT_COPY, -2) == _lib.ERR_ARGUMENT
+    assert b"TTVX_FAULT" in L.ttvx_last_error()
+    for site in (_lib.FAULT_ALLOC, _lib.FAULT_COPY, _lib.FAULT_LAUNCH):
+        assert L.ttvx_debug_fail_next(site, 2) == _lib.OK
+        assert L.ttvx_debug_fail_next(site, 0) == _lib.OK

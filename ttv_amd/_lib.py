@@ -17,6 +17,7 @@ NUM_FEATURES = 24
 
 # error codes
 OK, ERR_ARGUMENT, ERR_LAYOUT, ERR_DEVICE, ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+FAULT_ALLOC, FAULT_COPY, FAULT_LAUNCH = 0, 1, 2        # ttvx_debug_fail_next sites
 
 _sz = ctypes.c_size_t
 _szp = ctypes.POINTER(ctypes.c_size_t)
@@ -59,6 +60,7 @@ _SIGS = {
     "ttvx_bench_mix": ([_vp, _sz, _vp, _sz, _vp], _int),
     "ttvx_bench_sweep": ([_vp, _sz, _vp, _sz, _int, _vp, _vp], _int),
     "ttvx_kron_f32": ([_vp, _vp, _sz, _vp, _vp], _int),
+    "ttvx_debug_fail_next": ([_int, _int], _int),
     "ttvx_kron_f64": ([_vp, _vp, _sz, _vp, _vp], _int),
     "ttvx_bench_spin": ([_int, _vp, _vp], _int),
     "ttvx_mlp_work_floats": ([_int], _sz),

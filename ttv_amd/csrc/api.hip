@@ -54,6 +54,7 @@ struct Staged {
 // Device buffer of slot `slot` holding at least `bytes` (grown when too small).
 int stage_buffer(StageCache& cache, int slot, std::size_t bytes, void*& dev)
 {
+	if(injected(TTVX_FAULT_ALLOC)) return hip_fail(hipErrorOutOfMemory, "hipMalloc (staging)");
 	if(cache.cap[slot] < bytes) {
 		if(cache.buf[slot] != nullptr) (void)hipFree(cache.buf[slot]);
 		cache.buf[slot] = nullptr;
@@ -80,7 +81,8 @@ int stage_in(StageCache& cache, int slot, const void* host_or_dev, std::size_t b
 	if(rc != TTVX_OK) return rc;
 	s.staged = true;
 	if(copy) {
-		hipError_t e = hipMemcpy(s.dev, host_or_dev, bytes, hipMemcpyHostToDevice);
+		hipError_t e = injected(TTVX_FAULT_COPY) ? hipErrorInvalidValue
+		                                         : hipMemcpy(s.dev, host_or_dev, bytes, hipMemcpyHostToDevice);
 		if(e != hipSuccess) return hip_fail(e, "hipMemcpy (host to device)");
 	}
 	return TTVX_OK;
@@ -137,7 +139,7 @@ int ttv_sync(int code, std::size_t q, std::size_t p,
 		if(rc != TTVX_OK) return rc;
 	}
 	hipError_t e;
-	if(sc.staged) e = hipMemcpy(c, sc.dev, cb, hipMemcpyDeviceToHost);
+	if(sc.staged) e = injected(TTVX_FAULT_COPY) ? hipErrorInvalidValue : hipMemcpy(c, sc.dev, cb, hipMemcpyDeviceToHost);
 	else         e = hipStreamSynchronize(nullptr);
 	if(e != hipSuccess) return hip_fail(e, "completion of the product");
 	return TTVX_OK;
@@ -236,7 +238,7 @@ int ttvx_device_alloc(size_t bytes, void** out)
 	DeviceInfo di;
 	if(!current_device(di)) return TTVX_ERR_DEVICE;
 	if(bytes == 0) return TTVX_OK;
-	hipError_t const e = hipMalloc(out, bytes);
+	hipError_t const e = injected(TTVX_FAULT_ALLOC) ? hipErrorOutOfMemory : hipMalloc(out, bytes);
 	if(e != hipSuccess) {
 		*out = nullptr;
 		return hip_fail(e, "hipMalloc");
@@ -439,6 +441,15 @@ int ttvx_features(int dtype, size_t q, size_t p, const size_t* na, const size_t*
 	if(features == nullptr) return fail(TTVX_ERR_ARGUMENT, "ttvx: null output array");
 	Fold const f = fold(q, p, na, wa, pia, nullptr, nullptr);
 	features_of(f, dt, 256, features);
+	return TTVX_OK;
+}
+
+int ttvx_debug_fail_next(int site, int count)
+{
+	clear_error();
+	if(site < TTVX_FAULT_ALLOC || site > TTVX_FAULT_LAUNCH || count < 0)
+		return fail(TTVX_ERR_ARGUMENT, "ttvx_debug_fail_next: site must be a TTVX_FAULT_* value and count >= 0");
+	arm_fault(site, count);
 	return TTVX_OK;
 }
 

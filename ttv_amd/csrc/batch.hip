@@ -369,7 +369,7 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 			std::fill(owner + it.d.block0, owner + it.d.block0 + it.blocks, static_cast<uint32_t>(j));
 		}
 	}
-	hipError_t e = hipMalloc(&bt->dmem, std::max<std::size_t>(off, 256));
+	hipError_t e = injected(TTVX_FAULT_ALLOC) ? hipErrorOutOfMemory : hipMalloc(&bt->dmem, std::max<std::size_t>(off, 256));
 	if(e != hipSuccess) {
 		bt->dmem = nullptr;
 		delete bt;

@@ -39,6 +39,26 @@ int hip_fail(hipError_t e, const char* where)
 }
 const char* last_error_cstr() { return g_last_error.c_str(); }
 
+// ---- fault injection -------------------------------------------------------------------------
+namespace
+{
+std::atomic<int> g_faults[3];
+} // namespace
+
+void arm_fault(int site, int count)
+{
+	if(site >= 0 && site < 3) g_faults[site].store(count);
+}
+
+bool injected(int site)
+{
+	if(site < 0 || site >= 3) return false;
+	int v = g_faults[site].load();
+	while(v > 0)
+		if(g_faults[site].compare_exchange_weak(v, v - 1)) return true;
+	return false;
+}
+
 // ---- device --------------------------------------------------------------------------------
 namespace
 {
@@ -530,6 +550,7 @@ PartialBuffers g_partials[64];
 
 int partial_buffer(int device, hipStream_t stream, std::size_t bytes, void** out)
 {
+	if(injected(TTVX_FAULT_ALLOC)) return hip_fail(hipErrorOutOfMemory, "hipMalloc (split partials)");
 	PartialBuffers& pb = g_partials[device & 63];
 	std::lock_guard<std::mutex> lock(pb.mutex);
 	auto& slot = pb.by_stream[stream];
@@ -602,6 +623,7 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 	bool const sub4 = dt.size < 4 && cfg.kind == KIND_DOT && cfg.vec * dt.size >= 4 &&
 	                  ((reinterpret_cast<uintptr_t>(a) | static_cast<uint64_t>(f.sAo * dt.size) |
 	                    static_cast<uint64_t>(f.K * dt.size)) & 3u) != 0;
+	if(injected(TTVX_FAULT_LAUNCH)) return hip_fail(hipErrorLaunchFailure, "kernel launch");
 	int rc = TTVX_OK;
 	switch(dt.code) {
 		case TTVX_F32:     rc = launch_f32 (cfg, a, b, c, kahan, stream); break;

@@ -21,6 +21,12 @@ void clear_error();
 int  fail(int status, const std::string& msg);          // set_error + return status
 int  hip_fail(hipError_t e, const char* where);         // TTVX_ERR_DEVICE with HIP's text
 
+// ---- fault injection (test hook, ttvx_debug_fail_next) ---------------------------------------
+// True when a failure is armed at `site` (TTVX_FAULT_*); consumes one armed failure.
+bool injected(int site);
+// Arms `count` failures at `site` (replacing what was armed there).
+void arm_fault(int site, int count);
+
 // ---- device --------------------------------------------------------------------------------
 struct DeviceInfo {
 	int id = -1;
