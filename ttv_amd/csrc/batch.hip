@@ -103,7 +103,8 @@ using Key = std::tuple<int, int, int, int>;
 template<int VEC, int U>
 GroupKernel rows_kernel(int lprc)
 {
-	// compile-time lanes per row for full-width vectors, as the single-problem k_rows
+	// compile-time lanes per row for full-width vectors, as the single-problem k_rows; -1: tight rows
+	if(lprc < 0) return k_batch_group<BB_ROWS, VEC, U, -1>;
 	if constexpr(VEC == 4) {
 		switch(lprc) {
 			case 4:  return k_batch_group<BB_ROWS, VEC, U, 4>;
@@ -277,7 +278,8 @@ int ttvx_batch_create(size_t count, const ttvx_batch_problem* problems, void** o
 		} else if(cfg.lean == LEAN_ROWS) {
 			body = BB_ROWS;
 			it.blocks = (cfg.args.groups + kBlock / 64 - 1) / (kBlock / 64);
-			if(cfg.vec == 4 && cfg.args.LPR >= 4) lprc = cfg.args.LPR;
+			if(cfg.args.tight) lprc = -1;
+			else if(cfg.vec == 4 && cfg.args.LPR >= 4) lprc = cfg.args.LPR;
 		} else {
 			body = BB_DOT;
 			int64_t const waves = kBlock / 64;

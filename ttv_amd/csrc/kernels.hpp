@@ -253,8 +253,8 @@ struct Args {
 	int64_t iTiles;           // tiles along i (IT*VEC columns, or U*IT*VEC when along_i)
 	int64_t items;            // ceil(O / (OS*U)) * iTiles, or ceil(O / OS) * iTiles when along_i
 	// k_dot tiling
-	int LPR;                  // lanes per row (power of two <= 64)
-	int lprShift;             // log2(LPR)
+	int LPR;                  // lanes per row (a power of two <= 64; any 2..64 for tight k_rows)
+	int lprShift;             // log2(LPR) (power-of-two LPR)
 	int64_t groups;           // ceil(O / (rows per wave * U))
 	int kiter;                // vector steps per lane per row (for b caching)
 	// split of the contraction over gridDim.y
@@ -277,6 +277,8 @@ struct Args {
 	int itShift, spanShift;
 	// rows (I == 1) whose length the vector width does not divide: their last vector is clamped
 	int kpart;
+	// k_rows with LPR = ceil(K / VEC) lanes per row, not a power of two (rows_group, LPRC < 0)
+	int tight;
 };
 
 // Range of work units [begin, end) with step of block blockIdx.x (lane offset added by callers)
@@ -1101,20 +1103,76 @@ __device__ __forceinline__ void load_b_vec(const Acc* __restrict__ b, int64_t k0
 	}
 }
 
+// Sums each segment of `len` consecutive lanes (segment s: lanes s*len .. s*len+len-1) into the
+// segment's first lane, for U values per lane: shifts by 1, 2, 4, ... with the lanes past their
+// segment's end adding nothing.
+template<typename Acc, int U>
+__device__ __forceinline__ void segment_sum(Acc (&v)[U], int kl, int len)
+{
+	for(int d = 1; d < len; d <<= 1) {
+		bool const take = kl + d < len;
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			Acc const t = __shfl_down(v[u], d, 64);
+			v[u] += take ? t : Acc(0);
+		}
+	}
+}
+
 // Rows of K <= 64*VEC contiguous elements (one vector per lane and row, P.kiter == 1): wave
-// blockIdx.x*4 + w sums the U*64/LPR rows of its group and stores their sums.  Every load is issued
+// blockIdx.x*4 + w sums the U*RPW rows of its group and stores their sums.  Every load is issued
 // before the first use and none sits behind a branch: lanes past the end of a row re-read the row's
-// first vector and add nothing.  LPRC > 0 fixes the lanes per row at compile time (constant
-// cross-lane offsets); LPRC == 0 reads it from P.LPR.
+// first vector and add nothing.  LPRC > 0 fixes the lanes per row (a power of two) at compile time
+// (constant cross-lane offsets); LPRC == 0 reads the power of two from P.LPR; LPRC < 0 ("tight")
+// takes any P.LPR = ceil(K / VEC) lanes per row, RPW = 64 / LPR rows per wave, the lanes past RPW*LPR
+// idle, and sums rows by segmented shifts (9 lanes per row: 63 of 64 lanes busy instead of 9 of 16).
 template<typename S, int VEC, int U, int LPRC, bool KAHAN>
 __device__ __forceinline__ void rows_group(const S* __restrict__ A, const typename Traits<S>::acc* __restrict__ b,
                                            void* __restrict__ C, const Args& P, int64_t g)
 {
 	using Acc = typename Traits<S>::acc;
+	int const lane = static_cast<int>(threadIdx.x) & 63;
+	if constexpr(LPRC < 0) {
+		int const lpr = P.LPR;
+		int const rpw = 64 / lpr;
+		int const r   = lane / lpr;
+		int const kl  = lane - r * lpr;
+		int64_t const k0 = static_cast<int64_t>(kl) * VEC;
+		bool const live = r < rpw && k0 < P.K;
+		int64_t const kc = live ? row_clamp(k0, P.K, VEC) : 0;
+		Acc bb[VEC];
+		load_b_vec<Acc, VEC>(b, kc, bb);
+		Vec<S, VEC> raw[U];
+		int64_t const row0 = g * U * rpw + (r < rpw ? r : 0);
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			int64_t const row = row0 + static_cast<int64_t>(u) * rpw;
+			raw[u].load(A + (row < P.O ? row : P.O - 1) * P.sAo + kc);
+		}
+		Acc v[U];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			Sum<Acc, KAHAN> acc;
+			Acc x[VEC];
+			decode(raw[u], x);
+			if(P.kpart) mask_head<Acc, VEC>(x, k0 - kc);
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) acc.fma(x[e], bb[e]);
+			v[u] = live ? acc.value() : Acc(0);
+		}
+		segment_sum<Acc, U>(v, kl, lpr);
+		if(kl == 0 && r < rpw) {
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				int64_t const row = row0 + static_cast<int64_t>(u) * rpw;
+				if(row < P.O) store_out<Acc, kScaled<S>, (sizeof(S) >= 4)>(C, P, row, 0, v[u]);
+			}
+		}
+		return;
+	}
 	int const lpr   = LPRC > 0 ? LPRC : P.LPR;
 	int const shift = LPRC > 0 ? __builtin_ctz(LPRC) : P.lprShift;
 	int const rpw   = 64 >> shift;
-	int const lane = static_cast<int>(threadIdx.x) & 63;
 	int const r    = lane >> shift;
 	int64_t const k0 = static_cast<int64_t>(lane & (lpr - 1)) * VEC;
 	bool const live = k0 < P.K;

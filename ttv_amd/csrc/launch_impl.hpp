@@ -35,6 +35,10 @@ template<typename S, int VEC, int U, bool KH>
 int launch_rows(const Config& cfg, const S* A, const typename Traits<S>::acc* B, void* C, hipStream_t s)
 {
 	dim3 const grid(static_cast<unsigned>(cfg.grid)), block(kBlock);
+	if(cfg.args.tight) {
+		hipLaunchKernelGGL((k_rows<S, VEC, U, -1, KH>), grid, block, 0, s, A, B, C, cfg.args);
+		return TTVX_OK;
+	}
 	if constexpr(VEC == Traits<S>::vmax) {
 		switch(cfg.args.LPR) {
 			case 4:  hipLaunchKernelGGL((k_rows<S, VEC, U, 4, KH>), grid, block, 0, s, A, B, C, cfg.args); return TTVX_OK;

@@ -276,7 +276,17 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		cfg.ksplit = static_cast<int>(ksplit);
 		if(P.kiter == 1 && ksplit == 1 && ceil_div(P.groups, kBlock / 64) <= kMaxLeanBlocks) {
 			// rows of one vector per lane: k_rows, one group per wave and no cap on the grid (short-lived
-			// blocks handed out in address order sweep A as one compact front)
+			// blocks handed out in address order sweep A as one compact front).  When a power-of-two
+			// row group leaves many lanes idle (9 vectors per row: 9 of 16), rows take exactly their
+			// vectors' lanes ("tight": 7 rows of 9 lanes per wave) if that keeps >= 15 % more lanes busy
+			double const util_pow2 = static_cast<double>(vsteps) / P.LPR;
+			double const util_tight = static_cast<double>((64 / vsteps) * vsteps) / 64.0;
+			if(vsteps > 1 && !is_pow2(vsteps) && util_tight > 1.15 * util_pow2) {
+				P.tight = 1;
+				P.LPR = static_cast<int>(vsteps);
+				P.lprShift = 0;
+				P.groups = ceil_div(f.O, (64 / vsteps) * cfg.U);
+			}
 			cfg.lean = LEAN_ROWS;
 			cfg.grid = static_cast<int>(std::max<int64_t>(1, ceil_div(P.groups, kBlock / 64)));
 		} else {
