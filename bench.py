@@ -268,7 +268,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are out of scope (the mode-q product "
-              "shards over free modes with no exchange, see DESIGN.md)", file=sys.stderr)
+              "shards over free modes with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

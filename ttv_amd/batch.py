@@ -1,4 +1,4 @@
-"""Batched mode-q products in one persistent launch (ttvx_batch_* in ttvx.h).
+"""Batched mode-q products, one launch per group of problems sharing a tile body (ttvx_batch_* in ttvx.h).
 
     B = Batch([(q, A, b), ...])      # torch float32 GPU tensors; C tensors are allocated
     B.launch(graph=True)             # one launch per tile body (captured into a hipGraph on first use)

@@ -380,7 +380,7 @@ def main():
     ap.add_argument("--test", type=int, default=2000)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--out", default="gpurun_out/selector")
+    ap.add_argument("--out", default="selector_out")
     ap.add_argument("--tables", help="directory with timings_train.npz / timings_test.npz to train on")
     ap.add_argument("--retime-structured", action="store_true",
                     help="with --tables: time the benchmark shapes again first")
