@@ -120,10 +120,29 @@ bool read_table(const std::string& path, std::map<TKey, int>& table)
 	return !table.empty();
 }
 
+// The shipped weights and table were measured on gfx950: they apply when the current device is a
+// gfx950 or no GPU is visible (planning queries); elsewhere the fixed rule decides, unless the
+// environment names data explicitly.
+bool shipped_data_applies()
+{
+	int n = 0, dev = 0;
+	if(hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+		(void)hipGetLastError();
+		return true;
+	}
+	hipDeviceProp_t prop;
+	if(hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+		(void)hipGetLastError();
+		return true;
+	}
+	return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
 void load_default_selector_weights()
 {
 	std::lock_guard<std::mutex> lock(g_sel_mutex);
-	std::string const data = shipped_data_dir();
+	if(g_weights_decided && g_table_decided) return;
+	std::string const data = shipped_data_applies() ? shipped_data_dir() : std::string();
 	if(!g_weights_decided) {
 		g_weights_decided = true;
 		std::vector<std::string> paths;
