@@ -154,12 +154,18 @@ DType dtype_of(int code)
 
 namespace
 {
-// Vector width (elements, 16 bytes at most) of a row of `inner` elements: the widest power of two
-// that fits.  Vectors need only the element's alignment, and a row whose length the width does not
-// divide ends in a vector clamped to the row's end (kernels.hpp: col_clamp, row_clamp).
-int pick_vec(const DType& dt, int64_t inner)
+// Vector width (elements) of a row of `inner` elements: the widest power of two that fits, at most
+// 16 bytes.  Column kernels of FP8 storage take 8-byte vectors when 64 divides the row length, and
+// BF16 ones when the row is also at most 4096 elements: twice the lanes per row and half the
+// accumulators per lane (MI355X: FP8 rows of 128 columns +9-22 %, of 16384 +5-7 %; BF16 rows of 128
+// +4-11 %), while other row lengths keep 16 bytes (8 bytes lost 3-14 % on FP8 rows of 129-257
+// columns and 1-2 % on BF16 rows of 16384).  Vectors need only the element's alignment, and a row
+// whose length the width does not divide ends in a vector clamped to the row's end (kernels.hpp:
+// col_clamp, row_clamp).
+int pick_vec(const DType& dt, int64_t inner, bool columns)
 {
 	int v = dt.vmax;
+	if(columns && inner % 64 == 0 && (dt.size == 1 || (dt.size == 2 && inner <= 4096))) v = 8 / dt.size;
 	while(v > 1 && v > inner) v >>= 1;
 	return v;
 }
@@ -253,7 +259,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 	if(v >= V_DOT_U1 && v <= V_DOT_SPLIT) {
 		if(!dot_shape) return false;
 		cfg.kind = KIND_DOT;
-		cfg.vec  = pick_vec(dt, f.K);
+		cfg.vec  = pick_vec(dt, f.K, false);
 		int64_t const vsteps = ceil_div(f.K, cfg.vec);
 		P.LPR = static_cast<int>(std::min<int64_t>(64, pow2_ceil(vsteps)));
 		P.lprShift = ilog2(P.LPR);
@@ -300,7 +306,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE) {
 		if(!col_shape) return false;
 		cfg.kind = KIND_COL;
-		cfg.vec  = pick_vec(dt, f.I);
+		cfg.vec  = pick_vec(dt, f.I, true);
 		int64_t const iv = ceil_div(f.I, cfg.vec);          // vectors per row (the last one clamped)
 		int64_t const itmax = kBlock;
 		P.IT = static_cast<int>(std::min<int64_t>(iv, itmax));
