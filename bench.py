@@ -16,15 +16,15 @@ smaller ones rotate over copies), so every launch streams A from HBM.
 
 Also reported on the same line:
   roofline      -- the slowest of the 8 products: algorithmic bytes per launch / its mean launch
-                   time (K launches back to back) against the 8 TB/s HBM peak; traffic from the
-                   committed rocprofv3 counters
-  per_case      -- GB/s and fraction of peak for each (q, layout)
+                   time (K launches back to back, best of three such regions) against the 8 TB/s
+                   HBM peak; traffic from the committed rocprofv3 counters
+  per_case      -- GB/s and fraction of peak for each (q, layout), timed the same way
   read_ceiling  -- a pure 16-byte read stream over the same 1 GiB; mixed_ceilings -- the same read
                    stream plus a write stream of 1/128, 1/16 or 1/4 of it (the output fractions of
                    128^4, 16^7 and the order-8 q = 4 product); each the faster of a persistent
                    chunked stream and a compact sweep of short-lived blocks
   precision     -- BF16 and FP8-e4m3 storage of A: speedup over the FP32 kernel per case (both timed
-                   with the protocol above) and the relative L2 error of C against the FP64 oracle on
+                   with the per-case protocol) and the relative L2 error of C against the FP64 oracle on
                    the original data (and on the dequantised data)
   cpu_baseline  -- the reference's CPU path (restated in oracle/: OpenMP over slices + OpenBLAS
                    gemv, par_loop/subtensor/all) on the host's physical cores, on the same inputs
@@ -364,18 +364,23 @@ def main():
     step_bytes = sum(c["bytes"] for c in cases)
     value = step_bytes * args.steps / (total_ms * 1e-3) / 1e9
 
-    # per-case region: each product K times back to back between one event pair (its mean launch
-    # time); all 8 regions are enqueued behind one spin
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
-    spin(torch, ttv_amd, args.steps * len(cases) * 200.0, sink)
-    for c, (e0, e1) in zip(cases, evs):
-        e0.record(stream)
-        for s in range(args.steps):
-            run_case(c)
-        e1.record(stream)
-    torch.cuda.synchronize()
-    for c, (e0, e1) in zip(cases, evs):
-        c["ms"] = e0.elapsed_time(e1) / args.steps
+    # per-case regions: each product K times back to back between one event pair (its mean launch
+    # time); all 8 regions are enqueued behind one spin; the best of three such passes (a region can
+    # catch a stall of the box)
+    for c in cases:
+        c["ms"] = float("inf")
+    for _ in range(3):
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in cases]
+        spin(torch, ttv_amd, args.steps * len(cases) * 200.0, sink)
+        for c, (e0, e1) in zip(cases, evs):
+            e0.record(stream)
+            for s in range(args.steps):
+                run_case(c)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        for c, (e0, e1) in zip(cases, evs):
+            c["ms"] = min(c["ms"], e0.elapsed_time(e1) / args.steps)
+    for c in cases:
         c["gbs"] = c["bytes"] / (c["ms"] * 1e-3) / 1e9
     per_case = [dict(layout=c["layout"], q=c["q"], kernel=c["kernel"], ms=round(c["ms"], 4),
                      gbs=round(c["gbs"], 1), frac=round(c["gbs"] * 1e9 / PEAK_BPS, 4)) for c in cases]
@@ -716,7 +721,8 @@ def precision_leg(torch, np, ttv_amd, Aflat, a_host, cases, args, log):
                         for T in copies_for(flat)]
                 extra = scales.numel() * 4
                 elem = 1
-            ms = timed_cycle(torch, ttv_amd, runs, reps, sink, us_per_launch=150.0)
+            # best of three regions, like the FP32 per-case times (a region can catch a stall of the box)
+            ms = min(timed_cycle(torch, ttv_amd, runs, reps, sink, us_per_launch=150.0) for _ in range(3))
             got = Cq.as_strided([Cq.numel()], [1]).cpu().numpy().astype(np.float64)
             err = float(np.linalg.norm(got - ref) / rn)
             # error of the accumulation alone: FP64 oracle on the dequantised values

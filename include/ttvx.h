@@ -199,7 +199,7 @@ int ttvx_features(int dtype, size_t q, size_t p, const size_t* na, const size_t*
                   float* features);
 
 /* Number of floats in the selector MLP's parameter vector, and loading it (count 0 unloads).
- * Layout: mean[24], inv_std[24], W1[64][24], b1[64], W2[64][64], b2[64], W3[13][64], b3[13];
+ * Layout: mean[24], inv_std[24], W1[64][24], b1[64], W2[64][64], b2[64], W3[15][64], b3[15];
  * the network predicts log2(t_variant / t_fastest) for every variant. */
 size_t ttvx_selector_num_params(void);
 int ttvx_selector_set_weights(const float* params, size_t count);
@@ -213,7 +213,7 @@ int ttvx_selector_set_table(const int64_t* rows, size_t count);
 /* Device-pointer kernels that train and evaluate the selector MLP (parameter layout above).
  * work: ttvx_mlp_work_floats(batch) floats holding the activations of the last forward pass. */
 size_t ttvx_mlp_work_floats(int batch);
-/* y[batch][13] = MLP(x[batch][24]). */
+/* y[batch][15] = MLP(x[batch][24]). */
 int ttvx_mlp_forward(const float* params, const float* x, int batch, float* y, float* work, void* stream);
 /* Masked mean-squared error 0.5*sum(m*(y-t)^2)/batch into *loss and its gradient w.r.t. the
  * trainable parameters (everything after mean/inv_std, in parameter order) into grad. */

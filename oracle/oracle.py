@@ -372,7 +372,7 @@ def fiber_offsets(q, na, wa, pia):
 
 
 # ---- selector network (FP64 restatement of the layout in ttv_amd/csrc/mlp.hpp) ----------------
-MLP_F, MLP_H, MLP_V = 24, 64, 13
+MLP_F, MLP_H, MLP_V = 24, 64, 15
 
 
 def mlp_split(params):

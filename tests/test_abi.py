@@ -19,8 +19,8 @@ def test_exports_every_declared_symbol(ttvx):
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert L.ttvx_version() >= 10000
-    assert L.ttvx_num_variants() == 13
-    assert L.ttvx_selector_num_params() == 6653
+    assert L.ttvx_num_variants() == 15
+    assert L.ttvx_selector_num_params() == 6783
 
 
 def _call(ttvx, q, p, a, na, wa, pia, b, nb, c, nc, wc, pic):
@@ -141,12 +141,13 @@ def test_fold_general_strides(ttvx):
 
 def test_variants_and_features(ttvx):
     names = ttvx.variant_names()
-    assert len(names) == 13 and names[11] == "strided" and names[12] == "col_wide"
+    assert len(names) == 15 and names[11] == "strided" and names[12] == "col_wide"
+    assert names[13:] == ["col_u1_v8", "col_u2_v8"]
     na = [128] * 4
     for pia in ([1, 2, 3, 4], [4, 3, 2, 1]):
         wa = O.strides(na, pia)
         for q in range(1, 5):
-            valid = [v for v in range(13) if ttvx.variant_valid(q, na, wa, pia, v)]
+            valid = [v for v in range(len(names)) if ttvx.variant_valid(q, na, wa, pia, v)]
             assert 11 in valid
             info = ttvx.plan_info(q, na, wa, pia)
             assert info["variant"] in valid

@@ -64,7 +64,7 @@ def test_known_answer_grid_every_variant(ttvx, device, dt):
                         A = _dev_view(torch, device, a, na, wa)
                         b = torch.ones(na[q - 1], dtype=A.dtype, device=device)
                         exp = O.known_answer_expected(na[q - 1], int(np.prod(O.output_shape(na, q))))
-                        variants = [-1] + [v for v in range(13) if ttvx.variant_valid(q, na, wa, pia, v, code)]
+                        variants = [-1] + [v for v in range(ttvx.lib().ttvx_num_variants()) if ttvx.variant_valid(q, na, wa, pia, v, code)]
                         for v in variants:
                             if v < 0:
                                 ttvx.set_selection(ttvx.SELECT_AUTO)
@@ -125,7 +125,7 @@ def test_random_shapes_every_variant(ttvx, device):
             for q in range(1, p + 1):
                 b = rng.uniform(-1, 1, na[q - 1]).astype(dt)
                 ref = O.naive(q, a, na, wa, b, pia=pia)
-                for v in [v for v in range(13) if ttvx.variant_valid(q, na, wa, pia, v, code)]:
+                for v in [v for v in range(ttvx.lib().ttvx_num_variants()) if ttvx.variant_valid(q, na, wa, pia, v, code)]:
                     ttvx.set_selection(ttvx.SELECT_FORCED, v)
                     C = ttvx.ttv(q, A, torch.from_numpy(b).to(device))
                     assert _rel(_flat(C), ref) <= TOL[dt], (na, pia, q, v, dt)

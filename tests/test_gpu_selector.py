@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from ttv_amd.selector import NV
 
 pytestmark = pytest.mark.gpu
 
@@ -23,8 +24,8 @@ def test_forward_and_gradient_vs_oracle(ttvx, device):
     from ttv_amd.selector import GPUTrainer
     rng = np.random.default_rng(0)
     params, X = _params(rng)
-    T = rng.standard_normal((256, 13)).astype(np.float32)
-    M = (rng.uniform(size=(256, 13)) > 0.3).astype(np.float32)
+    T = rng.standard_normal((256, NV)).astype(np.float32)
+    M = (rng.uniform(size=(256, NV)) > 0.3).astype(np.float32)
     tr = GPUTrainer(params, 256)
     Xd, Td, Md = (torch.from_numpy(v).to(device) for v in (X, T, M))
     y = tr.forward(Xd).cpu().numpy()
@@ -42,7 +43,7 @@ def test_adam_reduces_loss(ttvx, device):
     from ttv_amd.selector import GPUTrainer
     rng = np.random.default_rng(1)
     params, X = _params(rng)
-    W = rng.standard_normal((24, 13)).astype(np.float32)
+    W = rng.standard_normal((24, NV)).astype(np.float32)
     T = np.maximum(X @ W * 0.1, 0).astype(np.float32)
     M = np.ones_like(T)
     tr = GPUTrainer(params, 256)
@@ -56,7 +57,7 @@ def test_adam_reduces_loss(ttvx, device):
 
 
 def test_gpu_and_host_inference_agree(ttvx, device):
-    from ttv_amd.selector import NV, predict_gpu
+    from ttv_amd.selector import predict_gpu
     from ttv_amd.workloads import selector_shapes
     rng = np.random.default_rng(2)
     params, _ = _params(rng)

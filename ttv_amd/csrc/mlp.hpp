@@ -17,7 +17,7 @@ namespace ttvx
 {
 constexpr int kMlpF = 24;   // TTVX_NUM_FEATURES
 constexpr int kMlpH = 64;
-constexpr int kMlpV = 13;   // number of kernel variants
+constexpr int kMlpV = 15;   // number of kernel variants (V_COUNT)
 constexpr std::size_t kMlpOffMean = 0;
 constexpr std::size_t kMlpOffStd  = kMlpOffMean + kMlpF;
 constexpr std::size_t kMlpOffW1   = kMlpOffStd + kMlpF;

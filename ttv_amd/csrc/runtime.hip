@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/ttvx.h"
+#include "mlp.hpp"
 
 namespace ttvx
 {
@@ -117,10 +118,12 @@ int check_device_span(const void* p, std::size_t bytes, const char* what)
 // ---- variants --------------------------------------------------------------------------------
 namespace
 {
+static_assert(kMlpV == V_COUNT, "the selector network has one output per kernel variant");
 const char* const kVariantNames[V_COUNT] = {
 	"dot_u1", "dot_u2", "dot_u4", "dot_u8", "dot_split",
 	"col_u1", "col_u2", "col_u4", "col_narrow", "col_split",
 	"col_k_lanes", "strided", "col_wide",
+	"col_u1_v8", "col_u2_v8",
 };
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -155,17 +158,12 @@ DType dtype_of(int code)
 namespace
 {
 // Vector width (elements) of a row of `inner` elements: the widest power of two that fits, at most
-// 16 bytes.  Column kernels of FP8 storage take 8-byte vectors when 64 divides the row length, and
-// BF16 ones when the row is also at most 4096 elements: twice the lanes per row and half the
-// accumulators per lane (MI355X: FP8 rows of 128 columns +9-22 %, of 16384 +5-7 %; BF16 rows of 128
-// +4-11 %), while other row lengths keep 16 bytes (8 bytes lost 3-14 % on FP8 rows of 129-257
-// columns and 1-2 % on BF16 rows of 16384).  Vectors need only the element's alignment, and a row
-// whose length the width does not divide ends in a vector clamped to the row's end (kernels.hpp:
-// col_clamp, row_clamp).
-int pick_vec(const DType& dt, int64_t inner, bool columns)
+// `bytes` bytes (16, or 8 for the col_*_v8 variants).  Vectors need only the element's alignment,
+// and a row whose length the width does not divide ends in a vector clamped to the row's end
+// (kernels.hpp: col_clamp, row_clamp).
+int pick_vec(const DType& dt, int64_t inner, int bytes = 16)
 {
-	int v = dt.vmax;
-	if(columns && inner % 64 == 0 && (dt.size == 1 || (dt.size == 2 && inner <= 4096))) v = 8 / dt.size;
+	int v = std::min(dt.vmax, bytes / dt.size);
 	while(v > 1 && v > inner) v >>= 1;
 	return v;
 }
@@ -259,7 +257,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 	if(v >= V_DOT_U1 && v <= V_DOT_SPLIT) {
 		if(!dot_shape) return false;
 		cfg.kind = KIND_DOT;
-		cfg.vec  = pick_vec(dt, f.K, false);
+		cfg.vec  = pick_vec(dt, f.K);
 		int64_t const vsteps = ceil_div(f.K, cfg.vec);
 		P.LPR = static_cast<int>(std::min<int64_t>(64, pow2_ceil(vsteps)));
 		P.lprShift = ilog2(P.LPR);
@@ -303,10 +301,15 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 			int64_t const per_wave = dt.size >= 4 ? kDotGroupsPerWave : 2 * kDotGroupsPerWave;
 			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, (kBlock / 64) * per_wave), dot_cap)));
 		}
-	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE) {
+	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE || v == V_COL_U1_V8 || v == V_COL_U2_V8) {
 		if(!col_shape) return false;
 		cfg.kind = KIND_COL;
-		cfg.vec  = pick_vec(dt, f.I, true);
+		// the _v8 variants: col_u1 / col_u2 with 8-byte lane vectors (twice the lanes per row, half the
+		// accumulators per lane; for 1- and 2-byte storage, where 16 bytes are 8-16 columns per lane)
+		bool const v8 = v == V_COL_U1_V8 || v == V_COL_U2_V8;
+		int const vb = v8 ? (v == V_COL_U1_V8 ? V_COL_U1 : V_COL_U2) : v;     // the tiling it follows
+		cfg.vec  = pick_vec(dt, f.I, v8 ? 8 : 16);
+		if(v8 && (dt.size > 2 || cfg.vec == pick_vec(dt, f.I))) return false;  // nothing narrower to take
 		int64_t const iv = ceil_div(f.I, cfg.vec);          // vectors per row (the last one clamped)
 		int64_t const itmax = kBlock;
 		P.IT = static_cast<int>(std::min<int64_t>(iv, itmax));
@@ -323,10 +326,10 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		}
 #endif
 		if(is_pow2(P.IT)) ks_deep = int64_t{1} << ilog2(ks_deep);     // power-of-two IT*KS <= 64 sums inside one wave
-		P.KS = static_cast<int>(v == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
+		P.KS = static_cast<int>(vb == V_COL_WIDE ? std::max<int64_t>(1, std::min<int64_t>(ks_cap, f.K)) : ks_deep);
 		P.OS = std::max(1, kBlock / (P.IT * P.KS));
 		static const int us[] = {1, 2, 4, 1, 1};
-		cfg.U = v == V_COL_WIDE ? TTVX_COL_WIDE_U : us[v - V_COL_U1];
+		cfg.U = vb == V_COL_WIDE ? TTVX_COL_WIDE_U : us[vb - V_COL_U1];
 		// wide rows (a full-width block still leaves U vectors per lane): the U units of a lane
 		// step along i, so a block streams U*IT*16 bytes of every row it touches
 		P.along_i = (cfg.U > 1 && iv >= static_cast<int64_t>(P.IT) * cfg.U && P.IT == kBlock) ? 1 : 0;
@@ -337,7 +340,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 			P.iTiles = ceil_div(iv, P.IT);
 			P.items  = ceil_div(f.O, static_cast<int64_t>(P.OS) * cfg.U) * P.iTiles;
 		}
-		if(v == V_COL_NARROW) {
+		if(vb == V_COL_NARROW) {
 			// k_slab: rows cut into chunks of IT <= kNarrowLanes vectors (a power of two), the block's
 			// other lanes along k (as many as K uses), OS slabs per block, one item per block
 			P.IT = static_cast<int>(std::min<int64_t>(int64_t{1} << ilog2(iv), kNarrowLanes));
@@ -359,7 +362,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 			return true;
 		}
 		int64_t ksplit = 1;
-		if(v == V_COL_SPLIT) {
+		if(vb == V_COL_SPLIT) {
 			// only when the output tiles cannot fill the GPU and every split keeps long k runs
 			int64_t const want = static_cast<int64_t>(std::max(cus, 1)) * 4;
 			if(P.items >= std::max(cus, 1) || f.K / P.KS < 64) return false;
@@ -372,7 +375,7 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		// one- and two-unit tiles: one item per block on an uncapped grid (short-lived blocks handed
 		// out in address order; MI355X, FP32: +1-2 % on 128^4 with I = 128, +4-7 % on 16^7 with
 		// I = 16..256 against blocks looping over items); wider tiles keep the capped grid
-		int64_t const cap = (v == V_COL_U1 || v == V_COL_U2) ? kMaxLeanBlocks : gridcap;
+		int64_t const cap = (vb == V_COL_U1 || vb == V_COL_U2) ? kMaxLeanBlocks : gridcap;
 		cfg.grid   = static_cast<int>(std::max<int64_t>(1, std::min(P.items, cap)));
 	} else if(v == V_COL_K_LANES) {
 		if(!dot_shape) return false;

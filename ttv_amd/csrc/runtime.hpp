@@ -48,6 +48,7 @@ enum Variant {
 	V_DOT_U1 = 0, V_DOT_U2, V_DOT_U4, V_DOT_U8, V_DOT_SPLIT,
 	V_COL_U1, V_COL_U2, V_COL_U4, V_COL_NARROW, V_COL_SPLIT,
 	V_COL_K_LANES, V_STRIDED, V_COL_WIDE,
+	V_COL_U1_V8, V_COL_U2_V8,         // col_u1 / col_u2 with 8-byte vectors (1- and 2-byte storage)
 	V_COUNT
 };
 const char* variant_name(int v);

@@ -1,7 +1,7 @@
 """Learned kernel selector: timing collection, GPU training and evaluation.
 
 The network (layout in ttvx.h / csrc/mlp.hpp) maps the 24 features of a problem
-(ttvx_features) to the predicted log2 slowdown of each of the 13 kernel variants relative to the
+(ttvx_features) to the predicted log2 slowdown of each of the 15 kernel variants relative to the
 fastest; the planner takes the valid variant with the smallest prediction.  The exhaustive
 autotuner is `collect_timings`: it times every valid variant of a shape.  Training runs on the
 GPU through the library's own kernels (ttvx_mlp_forward / _backward / _adam).
@@ -28,7 +28,7 @@ WEIGHTS_PATH = os.path.join(HERE, "data", "selector_gfx950.npz")
 WEIGHTS_BIN = os.path.join(HERE, "data", "selector_gfx950.bin")     # raw float32, loaded by libttvx itself
 TABLE_BIN = os.path.join(HERE, "data", "selector_table_gfx950.bin")  # raw int64 rows, loaded by libttvx itself
 STRUCT_WEIGHT = 8     # the benchmark shapes (equal extents, order 8) count this many times in the training loss
-NV = 13
+NV = 15
 NF = 24
 DT_CODE = {"f32": _lib.F32, "bf16": _lib.BF16, "f64": _lib.F64, "fp8": _lib.FP8E4M3}
 DT_NAME = {v: k for k, v in DT_CODE.items()}
@@ -362,10 +362,13 @@ def load_table(path=TABLE_BIN):
 
 
 def load_weights(path=WEIGHTS_PATH):
+    """The shipped weights, or None when absent or shaped for another network (another number of
+    kernel variants)."""
     if not os.path.exists(path):
         return None
     with np.load(path) as z:
-        return z["params"].astype(np.float32)
+        p = z["params"].astype(np.float32)
+    return p if p.size == int(_lib.lib().ttvx_selector_num_params()) else None
 
 
 def main():
