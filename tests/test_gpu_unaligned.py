@@ -1,8 +1,9 @@
 """Rows whose length the 16-byte vector width does not divide, and tensors that start off the
 16-byte grid, in all four storage types: the kernels keep full-width vector loads (element-aligned)
 and clamp the last vector of a row to the row's end.  Every valid kernel variant is forced on
-column problems (I = n*vmax + r, r = 1, 2, 3, vmax - 1) and row problems (K = n*vmax + r), with
-the tensor's base pointer 0, 1 and 3 elements past an aligned address; outputs must equal the FP64
+column problems (I = n*vmax + r, r = 1, 2, 3, vmax - 1; also I = 70*vmax + r, i-groups of a wave
+or more, where BF16/FP8 lanes pass realigned dwords to their neighbours) and row problems
+(K = n*vmax + r), with the tensor's base pointer 0, 1 and 3 elements past an aligned address; outputs must equal the FP64
 oracle on the values the kernel reads (FP32 1e-5, FP64 1e-12), and nothing outside C is written.
 The clamped vectors re-read elements that belong to the neighbouring lane; a non-finite value
 there must not reach another output (checked with an infinity planted next to a row end)."""
@@ -33,6 +34,7 @@ def _shapes(fmt):
         out.append(([n, 6, 4], [1, 2, 3], 1))      # rows: K = n contiguous
         out.append(([5, 9, n], [3, 2, 1], 1))      # columns in last-order layout: I = n (mode 3)
         out.append(([3, n * 9 + 1, 2], [2, 1, 3], 2))  # long rows: K = 9n + 1
+        out.append(([70 * v + r, 37, 3], [1, 2, 3], 2))  # slab rows wider than a wave of vectors
     return out
 
 

@@ -149,6 +149,46 @@ template<typename S, int VEC> struct Vec {
 	}
 };
 
+// Column loads of realigned storage without a second load per vector.  The lanes of an i-group hold
+// consecutive vectors of one slab row, so the dword after a lane's own W dwords (read from the
+// 4-byte boundary at or below its vector) is the first dword of the next lane: it comes over with one
+// ds_bpermute.  Lanes whose next lane holds no adjacent vector (the last lane of an i-group or of the
+// wave, lanes at the clamped end of a row) load that dword themselves, as realigned_load does.
+typedef uint32_t u32x2_a4x __attribute__((ext_vector_type(2), aligned(4)));
+template<typename S, int VEC> struct XVec {
+	static constexpr int W = Vec<S, VEC>::words;
+	uint32_t d[W];
+	uint32_t ext;
+	uint32_t m;
+	__device__ __forceinline__ void load(const S* p, bool edge)
+	{
+		uintptr_t const a = reinterpret_cast<uintptr_t>(p);
+		m = static_cast<uint32_t>(a & 3u);
+		uintptr_t const base = a - m;
+		if constexpr(W == 4) {
+			u32x4_a4 v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x4_a4, base));
+			d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+		} else if constexpr(W == 2) {
+			u32x2_a4x v = __builtin_nontemporal_load(TTVX_GLOBAL(u32x2_a4x, base));
+			d[0] = v.x; d[1] = v.y;
+		} else {
+			d[0] = __builtin_nontemporal_load(TTVX_GLOBAL(uint32_t, base));
+		}
+		ext = 0u;
+		if(edge) ext = __builtin_nontemporal_load(TTVX_GLOBAL(uint32_t, base + (m != 0u ? 4 * W : 4 * W - 4)));
+	}
+	// every lane of the i-group executes this together (the exchange reads the next lane's d[0])
+	__device__ __forceinline__ void finish(bool edge, int lane, Vec<S, VEC>& v) const
+	{
+		uint32_t const nxt = static_cast<uint32_t>(
+		    __builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, static_cast<int>(d[0])));
+		uint32_t const next = edge ? ext : nxt;
+#pragma unroll
+		for(int i = 0; i + 1 < W; ++i) v.w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], m);
+		v.w[W - 1] = __builtin_amdgcn_alignbyte(next, d[W - 1], m);
+	}
+};
+
 // decode element e of a loaded vector
 template<typename S, int VEC>
 __device__ __forceinline__ void decode(const Vec<S, VEC>& v, typename Traits<S>::acc (&x)[VEC]);
@@ -438,7 +478,7 @@ __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, i
 {
 	constexpr int BYTES = VEC * static_cast<int>(sizeof(Acc));
 	int64_t const i = iu;
-	if(ic != iu || P.partial != nullptr || P.cvec == 0 || i + VEC > P.I || (BYTES >= 16 ? P.cvec < 16 : P.cvec < BYTES)) {
+	if(ic != iu || P.partial != nullptr || i + VEC > P.I) {
 		// element stores; v[e] is column ic + e, and the columns before iu belong to the lane before
 #pragma unroll
 		for(int e = 0; e < VEC; ++e)
@@ -450,6 +490,24 @@ __device__ __forceinline__ void store_vec(void* __restrict__ C, const Args& P, i
 	for(int e = 0; e < VEC; ++e) w[e] = v[e];
 	if constexpr(SC) scale_vec<Acc, VEC>(P, o, i, w);
 	Acc* dst = static_cast<Acc*>(C) + o * P.sCo + i;
+	if(P.cvec == 0 || (BYTES >= 16 ? P.cvec < 16 : P.cvec < BYTES)) {
+		// C rows off the vector grid (a row length that 4 outputs do not divide): element-aligned
+		// vector stores (gfx950 takes them at the rate of aligned ones), the scales looked up once
+		// per vector as above
+		if constexpr(BYTES >= 16) {
+#pragma unroll
+			for(int c = 0; c < BYTES / 16; ++c) {
+				u32x4_a4 q;
+				uint32_t const* src = reinterpret_cast<const uint32_t*>(w) + 4 * c;
+				q.x = src[0]; q.y = src[1]; q.z = src[2]; q.w = src[3];
+				*TTVX_GLOBAL_W(u32x4_a4, dst + c * (16 / static_cast<int>(sizeof(Acc)))) = q;
+			}
+		} else {
+#pragma unroll
+			for(int e = 0; e < VEC; ++e) dst[e] = w[e];
+		}
+		return;
+	}
 	if constexpr(BYTES == 16) {
 		u32x4 q;
 		uint32_t const* src = reinterpret_cast<const uint32_t*>(w);
@@ -611,18 +669,26 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 		pa[u] = A + oc * P.sAo + (kb + ks) * P.sAk + ic;
 	}
 	Sum<Acc, KAHAN> acc[U][VEC];
+	// realigned storage: lanes whose next lane does not hold the following vector
+	bool edge[U];
+#pragma unroll
+	for(int u = 0; u < U; ++u) edge[u] = it == P.IT - 1 || (t & 63) == 63 || iu[u] + 2 * VEC > P.I;
 
 	if(active) {
 		int64_t k = kb + ks;
 		for(; k + (KU - 1) * P.KS < ke; k += KU * P.KS) {
 			Acc bk[KU];
 			Vec<S, VEC> raw[KU][U];
+			XVec<S, VEC> xr[KU][U];
 #pragma unroll
 			for(int j = 0; j < KU; ++j) bk[j] = b[k + j * P.KS];
 #pragma unroll
 			for(int j = 0; j < KU; ++j)
 #pragma unroll
-				for(int u = 0; u < U; ++u) raw[j][u].load(pa[u] + j * kstep);
+				for(int u = 0; u < U; ++u) {
+					if constexpr(kRealigned<S>) xr[j][u].load(pa[u] + j * kstep, edge[u]);
+					else raw[j][u].load(pa[u] + j * kstep);
+				}
 			// every load of the step is issued before any of them is used: without this fence the
 			// scheduler may trade loads in flight for registers and serialise them
 			__builtin_amdgcn_sched_barrier(0);
@@ -631,6 +697,7 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 #pragma unroll
 				for(int u = 0; u < U; ++u) {
 					Acc x[VEC];
+					if constexpr(kRealigned<S>) xr[j][u].finish(edge[u], t & 63, raw[j][u]);
 					decode(raw[j][u], x);
 #pragma unroll
 					for(int e = 0; e < VEC; ++e) acc[u][e].fma(x[e], bk[j]);
@@ -643,7 +710,13 @@ __device__ __forceinline__ void col_item(const S* __restrict__ A, const typename
 #pragma unroll
 			for(int u = 0; u < U; ++u) {
 				Vec<S, VEC> raw;
-				raw.load(pa[u]);
+				if constexpr(kRealigned<S>) {
+					XVec<S, VEC> xr;
+					xr.load(pa[u], edge[u]);
+					xr.finish(edge[u], t & 63, raw);
+				} else {
+					raw.load(pa[u]);
+				}
 				Acc x[VEC];
 				decode(raw, x);
 #pragma unroll

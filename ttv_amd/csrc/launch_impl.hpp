@@ -62,7 +62,7 @@ int launch_vec(const Config& cfg, const S* A, const typename Traits<S>::acc* B, 
 			hipLaunchKernelGGL((k_slab<S, VEC, KH>), grid, block, 0, s, A, B, C, cfg.args);
 			return TTVX_OK;
 		}
-		return fail(TTVX_ERR_UNSUPPORTED, "ttvx: realigned loads for row products only");
+		return fail(TTVX_ERR_UNSUPPORTED, "ttvx: realigned loads for row and column products only");
 	}
 	if(cfg.lean == LEAN_ROWS) {
 		switch(cfg.U) {
@@ -73,10 +73,6 @@ int launch_vec(const Config& cfg, const S* A, const typename Traits<S>::acc* B, 
 			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported row unroll");
 		}
 	}
-	if constexpr(kRealigned<S>) {
-		// realigned loads serve the row kernels only
-		if(cfg.kind != KIND_DOT) return fail(TTVX_ERR_UNSUPPORTED, "ttvx: realigned loads for row products only");
-	}
 	if(cfg.kind == KIND_DOT) {
 		switch(cfg.U) {
 			case 1: launch_sched(k_dot<S, VEC, 1, KH>, cfg, s, A, B, C, cfg.args); break;
@@ -85,7 +81,7 @@ int launch_vec(const Config& cfg, const S* A, const typename Traits<S>::acc* B, 
 			case 8: launch_sched(k_dot<S, VEC, 8, KH>, cfg, s, A, B, C, cfg.args); break;
 			default: return fail(TTVX_ERR_UNSUPPORTED, "ttvx: unsupported row unroll");
 		}
-	} else if constexpr(!kRealigned<S>) {
+	} else {
 		switch(cfg.U) {
 			case 1: launch_sched(k_col<S, VEC, 1, KH>, cfg, s, A, B, C, cfg.args); break;
 			case 2: launch_sched(k_col<S, VEC, 2, KH>, cfg, s, A, B, C, cfg.args); break;
@@ -113,7 +109,7 @@ int launch_kh(const Config& cfg, const S* A, const typename Traits<S>::acc* B, v
 			hipLaunchKernelGGL((k_strided<S, KH>), dim3(static_cast<unsigned>(cfg.grid)), dim3(kBlock), 0, s, A, B, C, cfg.sargs);
 			return TTVX_OK;
 		}
-		return fail(TTVX_ERR_UNSUPPORTED, "ttvx: realigned loads for row products only");
+		return fail(TTVX_ERR_UNSUPPORTED, "ttvx: realigned loads for row and column products only");
 	}
 	return launch_width<S, KH, Traits<S>::vmax>(cfg, A, B, C, s);
 }

@@ -634,14 +634,19 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 		cfg.args.fps    = fps > 0 ? fps : 1;
 	}
 
-	// 2- and 1-byte row products whose vectors do not all start on a 4-byte boundary run the
-	// realigned-load row kernels.  A row vector starts at o*sAo + k, k a multiple of the width or
-	// K - width for the clamped last one.  (Measured on MI355X, BF16 rows of 129 elements:
-	// 3.65 -> 4.29 TB/s, FP8: 2.68 -> 2.95 TB/s; column kernels lose 11-13 % with realigned loads
-	// and keep the unaligned ones.)
-	bool const sub4 = dt.size < 4 && cfg.kind == KIND_DOT && cfg.vec * dt.size >= 4 &&
-	                  ((reinterpret_cast<uintptr_t>(a) | static_cast<uint64_t>(f.sAo * dt.size) |
-	                    static_cast<uint64_t>(f.K * dt.size)) & 3u) != 0;
+	// 2- and 1-byte products whose vectors do not all start on a 4-byte boundary run the
+	// realigned-load kernels (loads from the 4-byte boundary, funnel-shifted):
+	// * rows: a row vector starts at o*sAo + k, k a multiple of the width or K - width for the
+	//   clamped last one (MI355X, BF16 rows of 129 elements: 3.65 -> 4.29 TB/s, FP8: 2.68 -> 2.95);
+	// * columns whose i-groups span at least a wave (the dword after a lane's vector comes from the
+	//   next lane): a slab row starts at o*sAo + k*sAk (MI355X, K = 16, I = 1025..65537: BF16
+	//   +2-8 %, FP8 +2-8 %).  Narrower i-groups keep the element-aligned loads: more of their lanes
+	//   sit at a group end and load the extra dword themselves (I = 129: -4 % BF16, -18 % FP8).
+	uint64_t const a_low = reinterpret_cast<uintptr_t>(a) | static_cast<uint64_t>(f.sAo * dt.size);
+	bool const sub4 = dt.size < 4 && cfg.vec * dt.size >= 4 &&
+	                  ((cfg.kind == KIND_DOT && ((a_low | static_cast<uint64_t>(f.K * dt.size)) & 3u) != 0) ||
+	                   (cfg.kind == KIND_COL && cfg.lean != LEAN_SLAB && !cfg.args.one_step && cfg.args.IT >= 64 &&
+	                    ((a_low | static_cast<uint64_t>(f.sAk * dt.size)) & 3u) != 0));
 	if(injected(TTVX_FAULT_LAUNCH)) return hip_fail(hipErrorLaunchFailure, "kernel launch");
 	int rc = TTVX_OK;
 	switch(dt.code) {
