@@ -1445,4 +1445,36 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const void* __restrict__ pa
 	}
 }
 
+// The same sums for many splits of few outputs: block j sums output j's partials (thread t takes
+// splits t, t + kBlock, ...; then the waves and the block, in a fixed order), in FP64.
+template<typename Acc>
+__global__ __launch_bounds__(kBlock) void k_finalize_wide(const void* __restrict__ partial, int part64, int splits,
+                                                          int64_t O, int64_t I, int64_t sCo, void* __restrict__ C,
+                                                          const float* __restrict__ scales, int64_t fps)
+{
+	__shared__ double red[kBlock / 64];
+	int64_t const n = O * I;
+	int const t = static_cast<int>(threadIdx.x);
+	for(int64_t j = blockIdx.x; j < n; j += gridDim.x) {
+		double s = 0.0;
+		for(int q = t; q < splits; q += kBlock)
+			s += part64 ? static_cast<const double*>(partial)[q * n + j]
+			            : static_cast<double>(static_cast<const Acc*>(partial)[q * n + j]);
+#pragma unroll
+		for(int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+		if((t & 63) == 0) red[t >> 6] = s;
+		__syncthreads();
+		if(t == 0) {
+			double sum = 0.0;
+#pragma unroll
+			for(int w = 0; w < kBlock / 64; ++w) sum += red[w];
+			Acc v = static_cast<Acc>(sum);
+			if(scales != nullptr) v *= static_cast<Acc>(scales[j / fps]);
+			int64_t const o = j / I, i = j - o * I;
+			static_cast<Acc*>(C)[o * sCo + i] = v;
+		}
+		__syncthreads();
+	}
+}
+
 } // namespace ttvx

@@ -14,6 +14,12 @@
 #include "../../include/ttvx.h"
 #include "mlp.hpp"
 
+// Waves per CU that the split row kernel aims for (its splits x row groups).  Measured on MI355X,
+// rows of 2^27-2^29 bytes, O = 2..100: 32 beats 8 and 16 by up to 30 % for 30 and more rows.
+#ifndef TTVX_SPLIT_WPC
+#define TTVX_SPLIT_WPC 32
+#endif
+
 namespace ttvx
 {
 
@@ -270,7 +276,10 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		int64_t ksplit = 1;
 		if(v == V_DOT_SPLIT) {
 			if(P.groups >= waves_wanted || vsteps < 4 * P.LPR) return false;  // nothing to gain
-			ksplit = std::min<int64_t>(64, ceil_div(waves_wanted, P.groups));
+			// up to 4096 splits: a few long rows (O = 2, K = 2^27) still fill the GPU with waves
+			// (0.7 -> 6.5 TB/s in FP32); their partials are summed by one block per output
+			// (k_finalize_wide)
+			ksplit = std::min<int64_t>(4096, ceil_div(static_cast<int64_t>(std::max(cus, 1)) * TTVX_SPLIT_WPC, P.groups));
 		}
 		int64_t kchunk = ceil_div(ceil_div(f.K, ksplit), kstride) * kstride;
 		ksplit = ceil_div(f.K, kchunk);
@@ -298,7 +307,8 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 			// hold more rows): enough waves to fill the GPU, and each wave still has a next group to load
 			// while it sums the current one
 			int64_t const dot_cap = static_cast<int64_t>(std::max(cus, 1)) * kDotGridPerCU;
-			int64_t const per_wave = dt.size >= 4 ? kDotGroupsPerWave : 2 * kDotGroupsPerWave;
+			// (the split variant: one group per wave, the splits were counted for that)
+			int64_t const per_wave = v == V_DOT_SPLIT ? 1 : (dt.size >= 4 ? kDotGroupsPerWave : 2 * kDotGroupsPerWave);
 			cfg.grid = static_cast<int>(std::max<int64_t>(1, std::min(ceil_div(P.groups, (kBlock / 64) * per_wave), dot_cap)));
 		}
 	} else if((v >= V_COL_U1 && v <= V_COL_SPLIT) || v == V_COL_WIDE || v == V_COL_U1_V8 || v == V_COL_U2_V8) {
@@ -659,7 +669,16 @@ int launch(const Fold& f, const DType& dt, const void* a, const void* b, void* c
 	if(rc == TTVX_OK && use_partial) {
 		int64_t const n = f.O * f.I;
 		unsigned const grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock), 4096)));
-		if(dt.code == TTVX_F64)
+		if(cfg.ksplit > 64) {
+			// many splits of few outputs: one block sums each output's partials
+			unsigned const gw = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(n, 4096)));
+			if(dt.code == TTVX_F64)
+				hipLaunchKernelGGL((k_finalize_wide<double>), dim3(gw), dim3(kBlock), 0, stream, partial, cfg.args.part64,
+				                   cfg.ksplit, f.O, f.I, f.sCo, c, scales, fps > 0 ? fps : 1);
+			else
+				hipLaunchKernelGGL((k_finalize_wide<float>), dim3(gw), dim3(kBlock), 0, stream, partial, cfg.args.part64,
+				                   cfg.ksplit, f.O, f.I, f.sCo, c, scales, fps > 0 ? fps : 1);
+		} else if(dt.code == TTVX_F64)
 			hipLaunchKernelGGL((k_finalize<double>), dim3(grid), dim3(kBlock), 0, stream, partial, cfg.args.part64,
 			                   cfg.ksplit, f.O, f.I, f.sCo, c, scales, fps > 0 ? fps : 1);
 		else
