@@ -174,11 +174,12 @@ def test_unaligned_and_strided_views(ttvx, device):
 
 def test_long_contractions_split_and_fp64_final(ttvx, device):
     """Few outputs with long fibers (split-n_q + two-pass reduction), and n_q > 2^20 (FP64 final
-    reduction), q innermost and outermost; the split row kernel also forced."""
+    reduction), q innermost and outermost; the split row and column kernels also forced."""
     import torch
     rng = np.random.default_rng(13)
     for na, pia, q in (([4, 300000], [2, 1], 2), ([3000, 7], [1, 2], 1), ([7, 3000], [1, 2], 2),
-                       ([2, (1 << 20) + 17], [2, 1], 2), ([(1 << 20) + 5, 3], [1, 2], 1)):
+                       ([2, (1 << 20) + 17], [2, 1], 2), ([(1 << 20) + 5, 3], [1, 2], 1),
+                       ([3, (1 << 20) + 9], [1, 2], 2)):
         wa = O.strides(na, pia)
         a = rng.uniform(-1, 1, O.span(na, wa)).astype(np.float32)
         b = rng.uniform(-1, 1, na[q - 1]).astype(np.float32)
@@ -187,14 +188,17 @@ def test_long_contractions_split_and_fp64_final(ttvx, device):
         for flags in (0, ttvx.FLAG_KAHAN, ttvx.FLAG_FP64_FINAL):
             C = ttvx.ttv(q, A, torch.from_numpy(b).to(device), flags=flags)
             assert _rel(_flat(C), ref) <= 1e-5, (na, pia, q, flags)
-        # the split row kernel forced: a few rows take hundreds of splits, summed one block per output
-        if ttvx.variant_valid(q, na, wa, pia, 4, ttvx.F32):
-            ttvx.set_selection(ttvx.SELECT_FORCED, 4)
+        # the split row (4) and column (9) kernels forced: a few outputs take hundreds of splits,
+        # summed one block per output
+        for v in (4, 9):
+            if not ttvx.variant_valid(q, na, wa, pia, v, ttvx.F32):
+                continue
+            ttvx.set_selection(ttvx.SELECT_FORCED, v)
             try:
                 for flags in (0, ttvx.FLAG_FP64_FINAL):
                     C = ttvx.ttv(q, A, torch.from_numpy(b).to(device), flags=flags)
                     torch.cuda.synchronize()
-                    assert _rel(_flat(C), ref) <= 1e-5, (na, pia, q, flags, "dot_split")
+                    assert _rel(_flat(C), ref) <= 1e-5, (na, pia, q, flags, v)
             finally:
                 ttvx.set_selection(ttvx.SELECT_AUTO)
 

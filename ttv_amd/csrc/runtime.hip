@@ -373,10 +373,12 @@ bool configure(int v, const Fold& f, const DType& dt, uintptr_t a_addr, int cus,
 		}
 		int64_t ksplit = 1;
 		if(vb == V_COL_SPLIT) {
-			// only when the output tiles cannot fill the GPU and every split keeps long k runs
-			int64_t const want = static_cast<int64_t>(std::max(cus, 1)) * 4;
+			// only when the output tiles cannot fill the GPU and every split keeps long k runs; up to
+			// 4096 splits (8 blocks per CU), so that a few narrow columns (O = 1, I = 2..64, K = 2^22
+			// and more) still fill the GPU
+			int64_t const want = static_cast<int64_t>(std::max(cus, 1)) * 8;
 			if(P.items >= std::max(cus, 1) || f.K / P.KS < 64) return false;
-			ksplit = std::min<int64_t>(64, std::min<int64_t>(ceil_div(want, P.items), f.K / P.KS / 32));
+			ksplit = std::min<int64_t>(4096, std::min<int64_t>(ceil_div(want, P.items), f.K / P.KS / 32));
 			if(ksplit < 2) return false;
 		}
 		int64_t const kchunk = ceil_div(f.K, ksplit);
